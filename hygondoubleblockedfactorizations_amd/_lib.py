@@ -1,0 +1,213 @@
+"""ctypes binding of ``libmifact.so`` (the C-ABI declared under ``include/``).
+
+The library is the product; this module only loads it and declares argument
+types.  Loading fails loudly when the library has not been built: there is no
+fallback implementation.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+
+PKG = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(PKG, "lib", "libmifact.so")
+
+P = ctypes.c_void_p
+I = ctypes.c_int
+
+# name -> argtypes (all functions return void unless listed in _RESTYPE)
+SIGNATURES = {
+    # mifact/handle.h
+    "mifact_create_handle": [ctypes.POINTER(P)],
+    "mifact_destroy_handle": [P],
+    "mifact_set_stream": [P, P],
+    "mifact_get_stream": [P, ctypes.POINTER(P)],
+    "mifact_set_error_mode": [P, I],
+    "mifact_get_last_error": [P],
+    "mifact_reserve_workspace": [P, I],
+    "mifact_version": [],
+    "mifact_profile_begin": [P],
+    "mifact_profile_end": [P, P, I, ctypes.POINTER(I)],
+    # potrf/double_blocked_potrf.h
+    "double_blocked_potrf": [P, P, I, I, I, I, P],
+    "double_blocked_potrf_dbtrsm": [P, P, I, I, I, I, P],
+    "double_blocked_potrf_kblocked_syrk": [P, P, I, I, I, I, I, P],
+    "double_blocked_potrf_kblocked_syrk_adaptive_gemm": [P, P, I, I, I, I, I, P],
+    "double_blocked_potrf_kblocked_syrk_adaptive_gemm_configured":
+        [P, P, I, I, I, I, I, I, I, I, P],
+    "double_blocked_potrf_double": [P, P, I, I, I, I, P],
+    "double_blocked_potrf_double_dbtrsm": [P, P, I, I, I, I, P],
+    "double_blocked_potrf_double_kblocked_syrk": [P, P, I, I, I, I, I, P],
+    "double_blocked_potrf_double_kblocked_syrk_adaptive_gemm": [P, P, I, I, I, I, I, P],
+    "double_blocked_potrf_double_kblocked_syrk_adaptive_gemm_configured":
+        [P, P, I, I, I, I, I, I, I, I, P],
+    # getrf/double_blocked_getrf.h
+    "double_blocked_getrf": [P, P, I, I, I, I, I, I, P, P],
+    "double_blocked_getrf_double": [P, P, I, I, I, I, I, I, P, P],
+    "double_blocked_getrf_batched_pivot": [P, P, I, I, I, I, I, I, I, P, P, P],
+    "double_blocked_getrf_double_batched_pivot": [P, P, I, I, I, I, I, I, I, P, P, P],
+    # gemm/update_gemm.h
+    "gemm_nt_update": [P, I, I, I, P, I, P, I, P, I],
+    "gemm_nt_update_double": [P, I, I, I, P, I, P, I, P, I],
+    "gemm_nt_update_m_tiled": [P, I, I, I, P, I, P, I, P, I, I],
+    "gemm_nt_update_m_tiled_double": [P, I, I, I, P, I, P, I, P, I, I],
+    "gemm_nn_update": [P, I, I, I, P, I, P, I, P, I],
+    "gemm_nn_update_double": [P, I, I, I, P, I, P, I, P, I],
+    # syrk/kblocked_syrk.h
+    "syrk_lower_update": [P, I, I, P, I, P, I],
+    "syrk_lower_update_double": [P, I, I, P, I, P, I],
+    "syrk_lower_update_kblocked": [P, I, I, P, I, P, I, I],
+    "syrk_lower_update_kblocked_double": [P, I, I, P, I, P, I, I],
+    # trsm/*.h
+    "trsm_right_lower_transpose": [P, I, I, P, I, P, I],
+    "trsm_right_lower_transpose_double": [P, I, I, P, I, P, I],
+    "double_blocked_trsm": [P, P, I, P, I, I, I, I, I],
+    "double_blocked_trsm_double": [P, P, I, P, I, I, I, I, I],
+    "trsm_left_lower_unit": [P, I, I, P, I, P, I],
+    "trsm_left_lower_unit_double": [P, I, I, P, I, P, I],
+    "trsm_left_lower_unit_iterative": [P, I, I, P, I, P, I, I],
+    "trsm_left_lower_unit_iterative_double": [P, I, I, P, I, P, I, I],
+    # kernels/panel_kernels.h
+    "mifact_potrf_block": [P, I, P, I, P],
+    "mifact_potrf_block_double": [P, I, P, I, P],
+    "mifact_getrf_panel": [P, I, I, P, I, P, P],
+    "mifact_getrf_panel_double": [P, I, I, P, I, P, P],
+    "mifact_isamax": [P, I, P, I, P],
+    "mifact_idamax": [P, I, P, I, P],
+}
+
+_RESTYPE = {
+    "mifact_create_handle": I,
+    "mifact_destroy_handle": I,
+    "mifact_set_stream": I,
+    "mifact_get_stream": I,
+    "mifact_set_error_mode": I,
+    "mifact_get_last_error": I,
+    "mifact_reserve_workspace": I,
+    "mifact_version": ctypes.c_char_p,
+    "mifact_profile_begin": I,
+    "mifact_profile_end": I,
+}
+
+_lib = None
+
+
+class LibraryNotBuilt(RuntimeError):
+    pass
+
+
+def load(path: str = LIB_PATH):
+    """Loads libmifact.so and declares every exported signature."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(path):
+        raise LibraryNotBuilt(
+            f"{path} is missing: build it with "
+            "`python -m hygondoubleblockedfactorizations_amd.build`")
+    lib = ctypes.CDLL(path)
+    for name, args in SIGNATURES.items():
+        fn = getattr(lib, name)
+        fn.argtypes = args
+        fn.restype = _RESTYPE.get(name, None)
+    _lib = lib
+    return lib
+
+
+class Handle:
+    """Owns a mifact_handle; binds it to a HIP stream (torch's by default)."""
+
+    def __init__(self, stream=None, error_mode_return=True):
+        lib = load()
+        self._lib = lib
+        h = P()
+        st = lib.mifact_create_handle(ctypes.byref(h))
+        if st != 0:
+            raise RuntimeError(f"mifact_create_handle failed with status {st}")
+        self.ptr = h
+        if error_mode_return:
+            lib.mifact_set_error_mode(self.ptr, 1)
+        if stream is not None:
+            self.set_stream(stream)
+
+    def set_stream(self, stream):
+        raw = getattr(stream, "cuda_stream", stream)
+        self._lib.mifact_set_stream(self.ptr, P(raw))
+
+    def last_error(self):
+        return self._lib.mifact_get_last_error(self.ptr)
+
+    def reserve(self, n):
+        st = self._lib.mifact_reserve_workspace(self.ptr, n)
+        if st != 0:
+            raise RuntimeError(f"workspace reservation failed: {st}")
+
+    def close(self):
+        if self.ptr:
+            self._lib.mifact_destroy_handle(self.ptr)
+            self.ptr = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+
+class StageStats(ctypes.Structure):
+    """Layout of mifact_stage_stats (include/mifact/handle.h)."""
+
+    _fields_ = [
+        ("name", ctypes.c_char * 32),
+        ("calls", I),
+        ("flops", ctypes.c_double),
+        ("bytes", ctypes.c_double),
+        ("ms", ctypes.c_double),
+    ]
+
+
+def profile_begin(handle):
+    load().mifact_profile_begin(handle.ptr)
+
+
+def profile_end(handle):
+    """Synchronizes and returns {stage: dict(calls, flops, bytes, ms)}."""
+    stats = (StageStats * 16)()
+    count = I(0)
+    st = load().mifact_profile_end(handle.ptr, ctypes.cast(stats, P), 16, ctypes.byref(count))
+    if st != 0:
+        raise RuntimeError(f"mifact_profile_end failed: {st}")
+    return {stats[i].name.decode(): dict(calls=stats[i].calls, flops=stats[i].flops,
+                                         bytes=stats[i].bytes, ms=stats[i].ms)
+            for i in range(count.value)}
+
+
+class BatchedPivotWorkspace(ctypes.Structure):
+    """Layout of getrf_batched_pivot_workspace (include/getrf/double_blocked_getrf.h)."""
+
+    _fields_ = [
+        ("device_sources", P),
+        ("device_targets", P),
+        ("device_swap_buffer", P),
+        ("host_pivots", P),
+        ("host_permutation", P),
+        ("host_sources", P),
+        ("host_targets", P),
+        ("capacity", I),
+        ("column_tile", I),
+    ]
+
+
+def ptr(t):
+    """Device/host address of a torch tensor or numpy array (None -> NULL)."""
+    if t is None:
+        return None
+    if hasattr(t, "data_ptr"):
+        return P(t.data_ptr())
+    return P(t.ctypes.data)

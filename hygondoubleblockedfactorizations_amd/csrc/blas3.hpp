@@ -1,0 +1,124 @@
+// Recursive BLAS-3 building blocks composed from the gfx950 kernels.
+#pragma once
+
+#include "kernels.hpp"
+
+namespace mifact {
+
+// Splits an order into two parts with the first a multiple of `align`.
+inline int split_point(int w, int align)
+{
+    int w1 = ((w / 2 + align - 1) / align) * align;
+    if (w1 >= w) w1 = w - align;
+    if (w1 <= 0) w1 = w / 2;
+    return w1;
+}
+
+// B(rows x cols) := B * L^{-T}, L lower non-unit (cols x cols).
+template <typename T>
+void trsm_rlt(mifact_handle h, int rows, int cols, const T* L, long long ldl, T* B,
+              long long ldb)
+{
+    if (rows <= 0 || cols <= 0) return;
+    if (cols <= 64) {
+        trsm_rlt_base<T>(h, rows, cols, L, ldl, B, ldb);
+        return;
+    }
+    const int c1 = split_point(cols, 64);
+    const int c2 = cols - c1;
+    trsm_rlt<T>(h, rows, c1, L, ldl, B, ldb);
+    gemm<T>(h, kOpNT, kShapeFull, rows, c2, c1, T(-1), B, ldb, at(L, ldl, c1, 0), ldl,
+            at(B, ldb, 0, c1), ldb);
+    trsm_rlt<T>(h, rows, c2, at(L, ldl, c1, c1), ldl, at(B, ldb, 0, c1), ldb);
+}
+
+// B(rows x cols) := L^{-1} B, L unit lower (rows x rows).
+template <typename T>
+void trsm_llu(mifact_handle h, int rows, int cols, const T* L, long long ldl, T* B,
+              long long ldb)
+{
+    if (rows <= 0 || cols <= 0) return;
+    if (rows <= 64) {
+        trsm_llu_base<T>(h, rows, cols, L, ldl, B, ldb);
+        return;
+    }
+    const int r1 = split_point(rows, 64);
+    const int r2 = rows - r1;
+    trsm_llu<T>(h, r1, cols, L, ldl, B, ldb);
+    gemm<T>(h, kOpNN, kShapeFull, r2, cols, r1, T(-1), at(L, ldl, r1, 0), ldl, B, ldb,
+            at(B, ldb, r1, 0), ldb);
+    trsm_llu<T>(h, r2, cols, at(L, ldl, r1, r1), ldl, at(B, ldb, r1, 0), ldb);
+}
+
+// Row-blocked variant: diagonal solves of `block` rows, each followed by a
+// GEMM update of the rows below (getrf_pivot_internal.h:64-110 schedule).
+template <typename T>
+void trsm_llu_iterative(mifact_handle h, int rows, int cols, const T* L, long long ldl,
+                        T* B, long long ldb, int block)
+{
+    if (rows <= 0 || cols <= 0) return;
+    if (block <= 0 || rows <= block) {
+        trsm_llu<T>(h, rows, cols, L, ldl, B, ldb);
+        return;
+    }
+    for (int row = 0; row < rows; row += block) {
+        const int dr = imin(block, rows - row);
+        trsm_llu<T>(h, dr, cols, at(L, ldl, row, row), ldl, B + row, ldb);
+        const int tr = rows - row - dr;
+        if (tr > 0) {
+            gemm<T>(h, kOpNN, kShapeFull, tr, cols, dr, T(-1), at(L, ldl, row + dr, row), ldl,
+                    B + row, ldb, B + row + dr, ldb);
+        }
+    }
+}
+
+// In-place lower Cholesky of an s x s diagonal block (any s): recursive
+// halving down to 64-wide leaves.  info: first failing global column.
+template <typename T>
+void potrf_diag(mifact_handle h, int s, T* A, long long lda, int* info, int col_offset)
+{
+    if (s <= 0) return;
+    if (s <= 64) {
+        potf2_base<T>(h, s, A, lda, info, col_offset);
+        return;
+    }
+    const int s1 = split_point(s, 64);
+    const int s2 = s - s1;
+    potrf_diag<T>(h, s1, A, lda, info, col_offset);
+    trsm_rlt<T>(h, s2, s1, A, lda, at(A, lda, s1, 0), lda);
+    gemm<T>(h, kOpNT, kShapeLower, s2, s2, s1, T(-1), at(A, lda, s1, 0), lda,
+            at(A, lda, s1, 0), lda, at(A, lda, s1, s1), lda);
+    potrf_diag<T>(h, s2, at(A, lda, s1, s1), lda, info, col_offset + s1);
+}
+
+// Recursive partial-pivot LU of rows [c0, n) x columns [c0, c0 + w) of A.
+// ipiv is written 1-based and global; orig tracks row contents; inv holds
+// kPermLevels snapshot arrays of n ints (level 0 belongs to the caller).
+template <typename T>
+void getrf_panel_rec(mifact_handle h, T* A, long long lda, int n, int c0, int w, int* ipiv,
+                     int* orig, int* inv, int nws, int* info, int level)
+{
+    if (w <= 0) return;
+    if (w <= 32) {
+        getf2_leaf<T>(h, A, lda, n, c0, w, ipiv, orig, info);
+        return;
+    }
+    if (level >= kPermLevels - 2) {
+        report_status(h, mifact_status_invalid_value, "getrf panel recursion too deep");
+        return;
+    }
+    const int w1 = split_point(w, 32);
+    const int w2 = w - w1;
+    int* inv_l = inv + static_cast<size_t>(level) * nws;
+    perm_snapshot(h, orig, inv_l, c0, n);
+    getrf_panel_rec<T>(h, A, lda, n, c0, w1, ipiv, orig, inv, nws, info, level + 1);
+    apply_row_moves<T>(h, A, lda, c0 + w1, c0 + w, c0, w1, ipiv, orig, inv_l, nullptr, 0);
+    trsm_llu<T>(h, w1, w2, at(A, lda, c0, c0), lda, at(A, lda, c0, c0 + w1), lda);
+    gemm<T>(h, kOpNN, kShapeFull, n - c0 - w1, w2, w1, T(-1), at(A, lda, c0 + w1, c0), lda,
+            at(A, lda, c0, c0 + w1), lda, at(A, lda, c0 + w1, c0 + w1), lda);
+    perm_snapshot(h, orig, inv_l, c0 + w1, n);
+    getrf_panel_rec<T>(h, A, lda, n, c0 + w1, w2, ipiv, orig, inv, nws, info, level + 1);
+    apply_row_moves<T>(h, A, lda, c0, c0 + w1, c0 + w1, w2, ipiv, orig, inv_l, nullptr, 0);
+}
+
+}  // namespace mifact

@@ -1,0 +1,142 @@
+// Internal definitions shared by the kernels and the host drivers.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstddef>
+#include <cstdint>
+#include <cstdio>
+#include <cstdlib>
+#include <vector>
+
+#include "mifact/handle.h"
+
+namespace mifact {
+
+// Stages of the stage profiler.
+enum Stage {
+    kStageGemm = 0,
+    kStagePotf2,
+    kStageTrsmLeaf,
+    kStageGetf2Leaf,
+    kStageRowMoves,
+    kStagePerm,
+    kStageCount
+};
+
+struct ProfileRecord {
+    int stage;
+    double flops;
+    double bytes;
+    hipEvent_t start;
+    hipEvent_t stop;
+};
+
+}  // namespace mifact
+
+struct mifact_handle_s {
+    hipStream_t stream = nullptr;
+    int device = 0;
+    int num_cu = 0;
+    int error_mode = mifact_error_exit;
+    mifact_status last_error = mifact_status_success;
+
+    // Device workspace arena: [ints | floats] sized for order ws_n.
+    void* ws = nullptr;
+    size_t ws_bytes = 0;
+    int ws_n = 0;
+
+    // Grid barrier counter of the cooperative panel kernel (device, 64-bit,
+    // monotonically increasing) plus the number of arrivals the host has
+    // issued so far (each launch waits for targets above this base).
+    unsigned long long* barrier = nullptr;
+    unsigned long long barrier_base = 0;
+    // Device status word written by kernels whose bounded waits expired.
+    int* kernel_status = nullptr;
+
+    // Stage profiler: event pool and the records of the current session.
+    bool profiling = false;
+    std::vector<hipEvent_t> event_pool;
+    size_t events_used = 0;
+    std::vector<mifact::ProfileRecord> records;
+};
+
+namespace mifact {
+
+// ---------------------------------------------------------------- errors
+
+void report_hip(mifact_handle h, hipError_t e, const char* what,
+                const char* file, int line);
+void report_status(mifact_handle h, mifact_status s, const char* what);
+
+#define MIFACT_HIP(h, call)                                                 \
+    do {                                                                    \
+        hipError_t e_ = (call);                                             \
+        if (e_ != hipSuccess) {                                             \
+            ::mifact::report_hip((h), e_, #call, __FILE__, __LINE__);       \
+            return;                                                         \
+        }                                                                   \
+    } while (0)
+
+#define MIFACT_HIP_RET(h, call, ret)                                        \
+    do {                                                                    \
+        hipError_t e_ = (call);                                             \
+        if (e_ != hipSuccess) {                                             \
+            ::mifact::report_hip((h), e_, #call, __FILE__, __LINE__);       \
+            return (ret);                                                   \
+        }                                                                   \
+    } while (0)
+
+// Checks the launch status of the last kernel.
+#define MIFACT_LAUNCHED(h) MIFACT_HIP((h), hipGetLastError())
+
+// ---------------------------------------------------------------- indexing
+
+// 64-bit column-major element offset: n = 65536 gives n^2 = 2^32 elements.
+__host__ __device__ inline size_t off(long long row, long long col, long long ld)
+{
+    return static_cast<size_t>(col) * static_cast<size_t>(ld)
+         + static_cast<size_t>(row);
+}
+
+template <typename T>
+__host__ __device__ inline T* at(T* A, long long ld, long long row, long long col)
+{
+    return A + off(row, col, ld);
+}
+
+inline int imin(int a, int b) { return a < b ? a : b; }
+inline int imax(int a, int b) { return a > b ? a : b; }
+inline int ceil_div(int a, int b) { return (a + b - 1) / b; }
+
+// ---------------------------------------------------------------- workspace
+
+// Fixed layout of the workspace arena for order n (all offsets in bytes).
+struct Workspace {
+    int* orig = nullptr;         // n ints: origin row of each row's content
+    int* inv = nullptr;          // kPermLevels x n ints: snapshots of orig^-1
+    int* moves = nullptr;        // 4 x n ints: move lists (src 2n | dst 2n)
+    float* exchange = nullptr;   // cooperative panel exchange records
+};
+
+constexpr int kPermLevels = 24;             // panel/outer + recursion levels
+constexpr size_t kExchangeBytes = 1 << 20;  // 2 parities x (G+1) records
+
+bool ensure_workspace(mifact_handle h, int n);
+Workspace workspace(mifact_handle h);
+
+// Brackets the kernel launches issued in its lifetime with an event pair
+// when the handle's profiler is on (no effect otherwise).
+class StageScope {
+public:
+    StageScope(mifact_handle h, Stage stage, double flops, double bytes);
+    ~StageScope();
+    StageScope(const StageScope&) = delete;
+    StageScope& operator=(const StageScope&) = delete;
+
+private:
+    mifact_handle h_;
+    long index_ = -1;
+};
+
+}  // namespace mifact

@@ -1,0 +1,71 @@
+// Host-side launchers of the gfx950 kernels (internal interface).
+//
+// Every launcher enqueues on h->stream and returns immediately; all matrices
+// are column-major with 64-bit leading dimensions.
+#pragma once
+
+#include "common.hpp"
+
+namespace mifact {
+
+enum GemmOp { kOpNT = 0, kOpNN = 1 };
+enum GemmShape { kShapeFull = 0, kShapeLower = 1 };
+
+// C(M x N) += alpha * A(M x K) * op(B)
+//   kOpNT: op(B) = B^T with B stored N x K   (C -= A B^T update: alpha = -1)
+//   kOpNN: op(B) = B   with B stored K x N
+// kShapeLower touches only C(i, j) with i >= j (SYRK / trapezoid updates).
+template <typename T>
+void gemm(mifact_handle h, GemmOp op, GemmShape shape, int M, int N, int K,
+          T alpha, const T* A, long long lda, const T* B, long long ldb, T* C,
+          long long ldc);
+
+// X * L^T = B for X (rows x s, s <= 64), L lower non-unit; B overwritten.
+template <typename T>
+void trsm_rlt_base(mifact_handle h, int rows, int s, const T* L, long long ldl,
+                   T* B, long long ldb);
+
+// L * X = B for X (s x cols, s <= 64), L lower unit; B overwritten.
+template <typename T>
+void trsm_llu_base(mifact_handle h, int s, int cols, const T* L, long long ldl,
+                   T* B, long long ldb);
+
+// Unblocked lower Cholesky of an s x s block (s <= 64) in one workgroup.
+// On a non-positive pivot at local column j: *info = col_offset + j + 1 if
+// *info was 0 (first failure in launch order wins).
+template <typename T>
+void potf2_base(mifact_handle h, int s, T* A, long long lda, int* info,
+                int col_offset);
+
+// Cooperative partial-pivot LU of a leaf panel: rows [c0, n) x columns
+// [c0, c0 + w) of the full matrix (w <= 32).  Writes ipiv[c0 + j] (global,
+// 1-based), carries orig[] (the origin of each row's content) along with the
+// row exchanges and records the first zero pivot in *info.
+template <typename T>
+void getf2_leaf(mifact_handle h, T* A, long long lda, int n, int c0, int w,
+                int* ipiv, int* orig, int* info);
+
+// inv[orig[p]] = p for p in [r0, n): snapshot of the row-origin map.
+void perm_snapshot(mifact_handle h, const int* orig, int* inv, int r0, int n);
+// orig[p] = p for p in [0, n).
+void perm_identity(mifact_handle h, int* orig, int n);
+// *info = 0 with a kernel (no memset nodes in captured graphs).
+void clear_int(mifact_handle h, int* p);
+
+// Applies the rows exchanged since a snapshot to a block of columns.
+// Candidate destination rows: [p0, p0 + K) and ipiv[p0 .. p0+K) - 1.
+// For each candidate pos: src = inv[orig[pos]]; if src != pos the column
+// entry at pos receives the (pre-call) entry at src.  Columns [c_begin,
+// c_end) of A.  tmp: optional global staging (2K x column chunk), used when
+// 2K values per column do not fit in LDS.
+template <typename T>
+void apply_row_moves(mifact_handle h, T* A, long long lda, int c_begin,
+                     int c_end, int p0, int K, const int* ipiv,
+                     const int* orig, const int* inv, T* tmp, long long tmp_elems);
+
+// Wave-reduction IAMAX: *result = 1-based index of the first entry of
+// maximum |x| (0 when n <= 0).
+template <typename T>
+void iamax(mifact_handle h, int n, const T* x, int incx, int* result);
+
+}  // namespace mifact

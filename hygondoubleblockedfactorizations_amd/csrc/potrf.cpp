@@ -1,0 +1,393 @@
+// Double-blocked POTRF driver and the BLAS-3 C-ABI entry points.
+//
+// Schedule restated from the reference (src/potrf/double_blocked_potrf.cpp:
+// double_blocked_potrf_impl :294-381, update_next_inner_panel :210-249,
+// prepare_next_outer_panel :251-292); the rocBLAS/rocSOLVER calls are
+// replaced by the gfx950 kernels through the recursive building blocks.
+#include <hip/hip_runtime.h>
+
+#include "blas3.hpp"
+#include "gemm/update_gemm.h"
+#include "kernels/panel_kernels.h"
+#include "potrf/double_blocked_potrf.h"
+#include "syrk/kblocked_syrk.h"
+#include "trsm/double_blocked_trsm.h"
+#include "trsm/left_lower_trsm.h"
+#include "trsm/right_lower_trsm.h"
+
+namespace mifact {
+namespace {
+
+constexpr int kDbTrsmMicroBlock = 1024;  // double_blocked_potrf.cpp:17
+
+// src/trsm/double_blocked_trsm.cpp:99-171: n rows of B, m columns of L.
+template <typename T>
+void double_blocked_trsm_impl(mifact_handle h, const T* L, long long ldl, T* B,
+                              long long ldb, int n, int m, int nb, int b)
+{
+    if (n <= 0 || m <= 0) return;
+    if (nb <= 0) nb = m;
+    if (b <= 0) b = nb;
+    b = imin(b, nb);
+    for (int outer = 0; outer < m; outer += nb) {
+        const int outer_size = imin(nb, m - outer);
+        const int outer_end = outer + outer_size;
+        for (int inner = outer; inner < outer_end; inner += b) {
+            const int inner_size = imin(b, outer_end - inner);
+            const int panel_start = inner + inner_size;
+            trsm_rlt<T>(h, n, inner_size, at(L, ldl, inner, inner), ldl, at(B, ldb, 0, inner), ldb);
+            if (panel_start < outer_end) {
+                const int pc = imin(b, outer_end - panel_start);
+                gemm<T>(h, kOpNT, kShapeFull, n, pc, panel_start - outer, T(-1),
+                        at(B, ldb, 0, outer), ldb, at(L, ldl, panel_start, outer), ldl,
+                        at(B, ldb, 0, panel_start), ldb);
+            }
+        }
+        const int trailing = m - outer_end;
+        if (trailing > 0) {
+            gemm<T>(h, kOpNT, kShapeFull, n, trailing, outer_size, T(-1), at(B, ldb, 0, outer),
+                    ldb, at(L, ldl, outer_end, outer), ldl, at(B, ldb, 0, outer_end), ldb);
+        }
+    }
+}
+
+template <typename T>
+void double_blocked_potrf_impl(mifact_handle h, T* A, int lda_i, int n, int nb, int b,
+                               int* info, bool use_dbtrsm)
+{
+    if (h == nullptr) {
+        report_status(nullptr, mifact_status_invalid_handle, "double_blocked_potrf: null handle");
+        return;
+    }
+    if (n <= 0) return;
+    if (nb <= 0) nb = n;
+    if (b <= 0) b = nb;
+    b = imin(b, nb);
+    const long long lda = lda_i;
+    if (info != nullptr) clear_int(h, info);
+
+    for (int outer = 0; outer < n; outer += nb) {
+        const int outer_size = imin(nb, n - outer);
+        const int outer_end = outer + outer_size;
+        for (int inner = outer; inner < outer_end; inner += b) {
+            const int inner_size = imin(b, outer_end - inner);
+            const int panel_start = inner + inner_size;
+            const int trailing_rows = n - panel_start;
+            T* Aii = at(A, lda, inner, inner);
+            // diagonal block (rocsolver_potrf at :334)
+            potrf_diag<T>(h, inner_size, Aii, lda, info, inner);
+            // panel solve (:336-354)
+            if (trailing_rows > 0) {
+                if (use_dbtrsm) {
+                    double_blocked_trsm_impl<T>(h, Aii, lda, at(A, lda, panel_start, inner), lda,
+                                                trailing_rows, inner_size, inner_size,
+                                                imin(kDbTrsmMicroBlock, inner_size));
+                } else {
+                    trsm_rlt<T>(h, trailing_rows, inner_size, Aii, lda,
+                                at(A, lda, panel_start, inner), lda);
+                }
+            }
+            // next inner panel -= L * L^T over the outer block's columns (:210-249);
+            // only its lower trapezoid is needed.
+            if (panel_start < outer_end) {
+                const int panel_cols = imin(b, outer_end - panel_start);
+                const int k = panel_start - outer;
+                gemm<T>(h, kOpNT, kShapeLower, n - panel_start, panel_cols, k, T(-1),
+                        at(A, lda, panel_start, outer), lda, at(A, lda, panel_start, outer), lda,
+                        at(A, lda, panel_start, panel_start), lda);
+            }
+        }
+        // next outer block -= L * L^T over all previous columns (:251-292):
+        // SYRK of the diagonal block + GEMM below it, fused in one
+        // lower-trapezoid launch.
+        const int next_outer = outer_end;
+        if (next_outer < n) {
+            const int next_size = imin(nb, n - next_outer);
+            gemm<T>(h, kOpNT, kShapeLower, n - next_outer, next_size, next_outer, T(-1),
+                    at(A, lda, next_outer, 0), lda, at(A, lda, next_outer, 0), lda,
+                    at(A, lda, next_outer, next_outer), lda);
+        }
+    }
+}
+
+template <typename T>
+void syrk_kblocked(mifact_handle h, int n, int k, const T* A, int lda, T* C, int ldc, int k_tile)
+{
+    if (n <= 0 || k <= 0) return;
+    const int step = (k_tile <= 0 || k_tile >= k) ? k : k_tile;
+    for (int k0 = 0; k0 < k; k0 += step) {
+        const int kc = imin(step, k - k0);
+        gemm<T>(h, kOpNT, kShapeLower, n, n, kc, T(-1), at(A, lda, 0, k0), lda,
+                at(A, lda, 0, k0), lda, C, ldc);
+    }
+}
+
+template <typename T>
+void gemm_nt_update_impl(mifact_handle h, int rows, int cols, int k, const T* A, int lda,
+                         const T* B, int ldb, T* C, int ldc)
+{
+    if (rows <= 0 || cols <= 0 || k <= 0) return;
+    gemm<T>(h, kOpNT, kShapeFull, rows, cols, k, T(-1), A, lda, B, ldb, C, ldc);
+}
+
+template <typename T>
+void gemm_nn_update_impl(mifact_handle h, int rows, int cols, int k, const T* A, int lda,
+                         const T* B, int ldb, T* C, int ldc)
+{
+    if (rows <= 0 || cols <= 0 || k <= 0) return;
+    gemm<T>(h, kOpNN, kShapeFull, rows, cols, k, T(-1), A, lda, B, ldb, C, ldc);
+}
+
+template <typename T>
+void potrf_block_impl(mifact_handle h, int n, T* A, int lda, int* info)
+{
+    if (info != nullptr) clear_int(h, info);
+    potrf_diag<T>(h, n, A, lda, info, 0);
+}
+
+template <typename T>
+void getrf_panel_impl(mifact_handle h, int m, int n, T* A, int lda, int* ipiv, int* info)
+{
+    if (m <= 0 || n <= 0) return;
+    if (m < n) {
+        report_status(h, mifact_status_invalid_value, "mifact_getrf_panel: needs m >= n");
+        return;
+    }
+    if (!ensure_workspace(h, m)) return;
+    Workspace ws = workspace(h);
+    if (info != nullptr) clear_int(h, info);
+    perm_identity(h, ws.orig, m);
+    getrf_panel_rec<T>(h, A, lda, m, 0, n, ipiv, ws.orig, ws.inv, h->ws_n, info, 1);
+}
+
+}  // namespace
+}  // namespace mifact
+
+using mifact::double_blocked_potrf_impl;
+
+extern "C" {
+
+void double_blocked_potrf(mifact_handle handle, float* A, int lda, int n, int nb, int b, int* info)
+{
+    double_blocked_potrf_impl<float>(handle, A, lda, n, nb, b, info, false);
+}
+
+void double_blocked_potrf_dbtrsm(mifact_handle handle, float* A, int lda, int n, int nb, int b,
+                                 int* info)
+{
+    double_blocked_potrf_impl<float>(handle, A, lda, n, nb, b, info, true);
+}
+
+void double_blocked_potrf_kblocked_syrk(mifact_handle handle, float* A, int lda, int n, int nb,
+                                        int b, int syrk_k_tile, int* info)
+{
+    (void)syrk_k_tile;
+    double_blocked_potrf_impl<float>(handle, A, lda, n, nb, b, info, false);
+}
+
+void double_blocked_potrf_kblocked_syrk_adaptive_gemm(mifact_handle handle, float* A, int lda,
+                                                      int n, int nb, int b, int syrk_k_tile,
+                                                      int* info)
+{
+    (void)syrk_k_tile;
+    double_blocked_potrf_impl<float>(handle, A, lda, n, nb, b, info, false);
+}
+
+void double_blocked_potrf_kblocked_syrk_adaptive_gemm_configured(
+    mifact_handle handle, float* A, int lda, int n, int nb, int b, int syrk_k_tile,
+    int gemm_m_tile, int gemm_m_threshold, int gemm_k_threshold, int* info)
+{
+    (void)syrk_k_tile;
+    (void)gemm_m_tile;
+    (void)gemm_m_threshold;
+    (void)gemm_k_threshold;
+    double_blocked_potrf_impl<float>(handle, A, lda, n, nb, b, info, false);
+}
+
+void double_blocked_potrf_double(mifact_handle handle, double* A, int lda, int n, int nb, int b,
+                                 int* info)
+{
+    double_blocked_potrf_impl<double>(handle, A, lda, n, nb, b, info, false);
+}
+
+void double_blocked_potrf_double_dbtrsm(mifact_handle handle, double* A, int lda, int n, int nb,
+                                        int b, int* info)
+{
+    double_blocked_potrf_impl<double>(handle, A, lda, n, nb, b, info, true);
+}
+
+void double_blocked_potrf_double_kblocked_syrk(mifact_handle handle, double* A, int lda, int n,
+                                               int nb, int b, int syrk_k_tile, int* info)
+{
+    (void)syrk_k_tile;
+    double_blocked_potrf_impl<double>(handle, A, lda, n, nb, b, info, false);
+}
+
+void double_blocked_potrf_double_kblocked_syrk_adaptive_gemm(mifact_handle handle, double* A,
+                                                             int lda, int n, int nb, int b,
+                                                             int syrk_k_tile, int* info)
+{
+    (void)syrk_k_tile;
+    double_blocked_potrf_impl<double>(handle, A, lda, n, nb, b, info, false);
+}
+
+void double_blocked_potrf_double_kblocked_syrk_adaptive_gemm_configured(
+    mifact_handle handle, double* A, int lda, int n, int nb, int b, int syrk_k_tile,
+    int gemm_m_tile, int gemm_m_threshold, int gemm_k_threshold, int* info)
+{
+    (void)syrk_k_tile;
+    (void)gemm_m_tile;
+    (void)gemm_m_threshold;
+    (void)gemm_k_threshold;
+    double_blocked_potrf_impl<double>(handle, A, lda, n, nb, b, info, false);
+}
+
+// ------------------------------------------------------------ BLAS-3 blocks
+
+void gemm_nt_update(mifact_handle handle, int rows, int cols, int k, const float* A, int lda,
+                    const float* B, int ldb, float* C, int ldc)
+{
+    mifact::gemm_nt_update_impl<float>(handle, rows, cols, k, A, lda, B, ldb, C, ldc);
+}
+
+void gemm_nt_update_double(mifact_handle handle, int rows, int cols, int k, const double* A,
+                           int lda, const double* B, int ldb, double* C, int ldc)
+{
+    mifact::gemm_nt_update_impl<double>(handle, rows, cols, k, A, lda, B, ldb, C, ldc);
+}
+
+void gemm_nt_update_m_tiled(mifact_handle handle, int rows, int cols, int k, const float* A,
+                            int lda, const float* B, int ldb, float* C, int ldc, int m_tile)
+{
+    (void)m_tile;
+    mifact::gemm_nt_update_impl<float>(handle, rows, cols, k, A, lda, B, ldb, C, ldc);
+}
+
+void gemm_nt_update_m_tiled_double(mifact_handle handle, int rows, int cols, int k,
+                                   const double* A, int lda, const double* B, int ldb,
+                                   double* C, int ldc, int m_tile)
+{
+    (void)m_tile;
+    mifact::gemm_nt_update_impl<double>(handle, rows, cols, k, A, lda, B, ldb, C, ldc);
+}
+
+void gemm_nn_update(mifact_handle handle, int rows, int cols, int k, const float* L, int ldl,
+                    const float* U, int ldu, float* C, int ldc)
+{
+    mifact::gemm_nn_update_impl<float>(handle, rows, cols, k, L, ldl, U, ldu, C, ldc);
+}
+
+void gemm_nn_update_double(mifact_handle handle, int rows, int cols, int k, const double* L,
+                           int ldl, const double* U, int ldu, double* C, int ldc)
+{
+    mifact::gemm_nn_update_impl<double>(handle, rows, cols, k, L, ldl, U, ldu, C, ldc);
+}
+
+void syrk_lower_update(mifact_handle handle, int n, int k, const float* A, int lda, float* C,
+                       int ldc)
+{
+    mifact::syrk_kblocked<float>(handle, n, k, A, lda, C, ldc, 0);
+}
+
+void syrk_lower_update_double(mifact_handle handle, int n, int k, const double* A, int lda,
+                              double* C, int ldc)
+{
+    mifact::syrk_kblocked<double>(handle, n, k, A, lda, C, ldc, 0);
+}
+
+void syrk_lower_update_kblocked(mifact_handle handle, int n, int k, const float* A, int lda,
+                                float* C, int ldc, int k_tile)
+{
+    mifact::syrk_kblocked<float>(handle, n, k, A, lda, C, ldc, k_tile);
+}
+
+void syrk_lower_update_kblocked_double(mifact_handle handle, int n, int k, const double* A,
+                                       int lda, double* C, int ldc, int k_tile)
+{
+    mifact::syrk_kblocked<double>(handle, n, k, A, lda, C, ldc, k_tile);
+}
+
+void trsm_right_lower_transpose(mifact_handle handle, int rows, int cols, const float* A, int lda,
+                                float* B, int ldb)
+{
+    mifact::trsm_rlt<float>(handle, rows, cols, A, lda, B, ldb);
+}
+
+void trsm_right_lower_transpose_double(mifact_handle handle, int rows, int cols, const double* A,
+                                       int lda, double* B, int ldb)
+{
+    mifact::trsm_rlt<double>(handle, rows, cols, A, lda, B, ldb);
+}
+
+void double_blocked_trsm(mifact_handle handle, float* A, int lda, float* B, int ldb, int n, int m,
+                         int nb, int b)
+{
+    mifact::double_blocked_trsm_impl<float>(handle, A, lda, B, ldb, n, m, nb, b);
+}
+
+void double_blocked_trsm_double(mifact_handle handle, double* A, int lda, double* B, int ldb,
+                                int n, int m, int nb, int b)
+{
+    mifact::double_blocked_trsm_impl<double>(handle, A, lda, B, ldb, n, m, nb, b);
+}
+
+void trsm_left_lower_unit(mifact_handle handle, int rows, int cols, const float* L, int ldl,
+                          float* B, int ldb)
+{
+    mifact::trsm_llu<float>(handle, rows, cols, L, ldl, B, ldb);
+}
+
+void trsm_left_lower_unit_double(mifact_handle handle, int rows, int cols, const double* L,
+                                 int ldl, double* B, int ldb)
+{
+    mifact::trsm_llu<double>(handle, rows, cols, L, ldl, B, ldb);
+}
+
+void trsm_left_lower_unit_iterative(mifact_handle handle, int rows, int cols, const float* L,
+                                    int ldl, float* B, int ldb, int block_size)
+{
+    mifact::trsm_llu_iterative<float>(handle, rows, cols, L, ldl, B, ldb, block_size);
+}
+
+void trsm_left_lower_unit_iterative_double(mifact_handle handle, int rows, int cols,
+                                           const double* L, int ldl, double* B, int ldb,
+                                           int block_size)
+{
+    mifact::trsm_llu_iterative<double>(handle, rows, cols, L, ldl, B, ldb, block_size);
+}
+
+// ------------------------------------------------------------ panel kernels
+
+void mifact_potrf_block(mifact_handle handle, int n, float* A, int lda, int* info)
+{
+    mifact::potrf_block_impl<float>(handle, n, A, lda, info);
+}
+
+void mifact_potrf_block_double(mifact_handle handle, int n, double* A, int lda, int* info)
+{
+    mifact::potrf_block_impl<double>(handle, n, A, lda, info);
+}
+
+void mifact_getrf_panel(mifact_handle handle, int m, int n, float* A, int lda, int* ipiv,
+                        int* info)
+{
+    mifact::getrf_panel_impl<float>(handle, m, n, A, lda, ipiv, info);
+}
+
+void mifact_getrf_panel_double(mifact_handle handle, int m, int n, double* A, int lda, int* ipiv,
+                               int* info)
+{
+    mifact::getrf_panel_impl<double>(handle, m, n, A, lda, ipiv, info);
+}
+
+void mifact_isamax(mifact_handle handle, int n, const float* x, int incx, int* result)
+{
+    mifact::iamax<float>(handle, n, x, incx, result);
+}
+
+void mifact_idamax(mifact_handle handle, int n, const double* x, int incx, int* result)
+{
+    mifact::iamax<double>(handle, n, x, incx, result);
+}
+
+}  // extern "C"

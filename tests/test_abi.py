@@ -1,0 +1,93 @@
+"""The C-ABI library loads and exports every function include/*.h declares.
+
+No compute calls: this runs without a GPU.
+"""
+import glob
+import os
+import re
+import subprocess
+
+import pytest
+
+from hygondoubleblockedfactorizations_amd import _lib
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def declared_functions():
+    names = set()
+    for path in glob.glob(os.path.join(ROOT, "include", "**", "*.h"), recursive=True):
+        text = open(path).read()
+        text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+        text = re.sub(r"//[^\n]*", "", text)
+        text = re.sub(r"typedef[^;]*?\{.*?\}[^;]*;", "", text, flags=re.S)
+        text = re.sub(r"typedef[^;]*;", "", text)
+        for m in re.finditer(r"([A-Za-z_][A-Za-z0-9_]*)\s*\(", text):
+            name = m.group(1)
+            if name in ("if", "for", "while", "sizeof", "extern"):
+                continue
+            names.add(name)
+    return names
+
+
+def exported_symbols():
+    out = subprocess.run(["nm", "-D", "--defined-only", _lib.LIB_PATH],
+                         capture_output=True, text=True, check=True).stdout
+    return {line.split()[-1] for line in out.splitlines() if line.strip()}
+
+
+def test_library_loads():
+    lib = _lib.load()
+    assert lib.mifact_version().startswith(b"mifact")
+
+
+def test_every_declared_symbol_is_exported():
+    declared = declared_functions()
+    assert len(declared) >= 40
+    missing = declared - exported_symbols()
+    assert not missing, sorted(missing)
+
+
+def test_binding_table_matches_headers():
+    assert set(_lib.SIGNATURES) == declared_functions()
+
+
+def test_reference_entry_points_present():
+    # The reference's public driver entry points
+    # (include/potrf/double_blocked_potrf.h:9-103,
+    #  include/getrf/double_blocked_getrf.h:26-74).
+    ref = {
+        "double_blocked_potrf", "double_blocked_potrf_dbtrsm",
+        "double_blocked_potrf_kblocked_syrk",
+        "double_blocked_potrf_kblocked_syrk_adaptive_gemm",
+        "double_blocked_potrf_kblocked_syrk_adaptive_gemm_configured",
+        "double_blocked_potrf_double", "double_blocked_potrf_double_dbtrsm",
+        "double_blocked_potrf_double_kblocked_syrk",
+        "double_blocked_potrf_double_kblocked_syrk_adaptive_gemm",
+        "double_blocked_potrf_double_kblocked_syrk_adaptive_gemm_configured",
+        "double_blocked_getrf", "double_blocked_getrf_double",
+        "double_blocked_getrf_batched_pivot",
+        "double_blocked_getrf_double_batched_pivot",
+        "gemm_nt_update", "gemm_nt_update_double", "gemm_nt_update_m_tiled",
+        "gemm_nt_update_m_tiled_double", "syrk_lower_update",
+        "syrk_lower_update_double", "syrk_lower_update_kblocked",
+        "syrk_lower_update_kblocked_double", "trsm_right_lower_transpose",
+        "trsm_right_lower_transpose_double", "double_blocked_trsm",
+        "double_blocked_trsm_double",
+    }
+    assert not (ref - exported_symbols())
+
+
+def test_no_rocblas_dependency():
+    out = subprocess.run(["ldd", _lib.LIB_PATH], capture_output=True, text=True).stdout
+    assert "rocblas" not in out and "rocsolver" not in out
+
+
+def test_missing_library_fails_loudly(tmp_path):
+    saved = _lib._lib
+    try:
+        _lib._lib = None
+        with pytest.raises(_lib.LibraryNotBuilt):
+            _lib.load(str(tmp_path / "absent.so"))
+    finally:
+        _lib._lib = saved
