@@ -1,0 +1,316 @@
+#!/usr/bin/env python3
+"""POTRF / GETRF FP32 throughput on one MI355X (n = 32768, column-major, lda = n).
+
+One step = one double-blocked POTRF of an SPD matrix (A = G G^T / n + I,
+G iid N(0,1)) plus one double-blocked partial-pivot GETRF of an iid
+Uniform(-1, 1) matrix, both n x n FP32, resident in HBM before timing.
+Inputs are reset (device-to-device copy) outside the timed region before
+every factorization; each factorization is bracketed by synchronize + HIP
+events on the library's stream.  value = (n^3/3 + 2 n^3/3) / summed time.
+
+Also reported:
+  * roofline of the dominant kernel (the MFMA update GEMM), from the
+    library's stage profiler (HIP event pairs around every launch, on the
+    launch stream, over the timed steps);
+  * cpu_baseline: LAPACK spotrf + sgetrf (the oracle's LAPACK loader,
+    OpenBLAS) on host cores, on a bounded sample of the same input families;
+  * the FP64 residual ||A - L L^T||_F/||A||_F and ||P A - L U||_F/||A||_F of
+    the last factorization, computed on the GPU in FP64 (check only).
+
+  python bench.py [--gpus 1] [--steps K] [--warmup W] [--n 32768]
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+METRIC = "POTRF/GETRF TFLOP/s @n=32768; mixed+IR vs FP32 solve time, same accuracy"
+FP32_PEAK_TFLOPS = 157.3      # MI355X dense FP32 (MFMA)
+HBM_PEAK_GBS = 8000.0
+
+
+def log(*a):
+    print("[bench]", *a, file=sys.stderr, flush=True)
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--n", type=int, default=32768)
+    ap.add_argument("--potrf-nb", type=int, default=0)
+    ap.add_argument("--potrf-b", type=int, default=0)
+    ap.add_argument("--getrf-nb", type=int, default=0)
+    ap.add_argument("--getrf-b", type=int, default=0)
+    ap.add_argument("--trsm-block", type=int, default=0)
+    ap.add_argument("--routines", default="potrf,getrf")
+    ap.add_argument("--seed", type=int, default=20260716)
+    ap.add_argument("--cpu-n", type=int, default=16384)
+    ap.add_argument("--cpu-threads", type=int, default=0)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-verify", action="store_true")
+    ap.add_argument("--no-profile", action="store_true")
+    return ap.parse_args()
+
+
+def block_sizes(args, n):
+    """Block sizes: explicit flags, else the autotuned table, else the
+    reference's published policy (docs/numerical_validation.md:62-66)."""
+    pnb, pb = args.potrf_nb, args.potrf_b
+    gnb, gb, tb = args.getrf_nb, args.getrf_b, args.trsm_block
+    try:
+        from hygondoubleblockedfactorizations_amd import tuning
+        t = tuning.lookup(n)
+    except Exception:
+        t = None
+    if t:
+        pnb = pnb or t["potrf_nb"]
+        pb = pb or t["potrf_b"]
+        gnb = gnb or t["getrf_nb"]
+        gb = gb or t["getrf_b"]
+        tb = tb or t["trsm_block"]
+    pnb = pnb or min(8192, n)
+    pb = pb or min(1024, pnb)
+    gnb = gnb or min(n // 2 if n >= 2 else n, 20480) or n
+    gb = gb or min(max(n // 4, 1), 4096)
+    tb = tb or 2048
+    return pnb, pb, gnb, gb, tb
+
+
+def gen_spd(torch, n, seed):
+    """A = G G^T / n + I in FP32 on the GPU (exactly symmetric)."""
+    g = torch.Generator(device="cuda")
+    g.manual_seed(seed)
+    G = torch.randn(n, n, generator=g, device="cuda", dtype=torch.float32)
+    A = torch.mm(G, G.T)
+    del G
+    A.div_(float(n))
+    A = torch.tril(A)
+    A = A + torch.tril(A, -1).T
+    A.diagonal().add_(1.0)
+    return A.contiguous()
+
+
+def gen_uniform(torch, n, seed):
+    g = torch.Generator(device="cuda")
+    g.manual_seed(seed)
+    A = torch.rand(n, n, generator=g, device="cuda", dtype=torch.float32)
+    return A.mul_(2.0).sub_(1.0)
+
+
+def potrf_residual_gpu(torch, A_orig, L_work):
+    """||A - L L^T||_F / ||A||_F, FP64 products on the GPU (column-major view)."""
+    M = A_orig.T.double()
+    L = torch.tril(L_work.T).double()
+    R = M - L @ L.T
+    return float(torch.linalg.norm(R) / torch.linalg.norm(M))
+
+
+def getrf_residual_gpu(torch, A_orig, LU_work, ipiv):
+    import numpy as np
+    n = A_orig.shape[0]
+    perm = np.arange(n)
+    piv = ipiv.cpu().numpy()[:n] - 1
+    for k in range(n):
+        p = piv[k]
+        if p != k:
+            perm[k], perm[p] = perm[p], perm[k]
+    M = A_orig.T.double()
+    PA = M[torch.from_numpy(perm).cuda()]
+    X = LU_work.T.double()
+    L = torch.tril(X, -1)
+    L.diagonal().add_(1.0)
+    U = torch.triu(X)
+    R = PA - L @ U
+    return float(torch.linalg.norm(R) / torch.linalg.norm(M))
+
+
+def cpu_baseline(torch, args):
+    """LAPACK spotrf + sgetrf on host cores, bounded sample (n = cpu_n)."""
+    import numpy as np
+    from oracle import oracle as O
+    path, config = O.lapack_open()
+    threads = args.cpu_threads or min(16, len(os.sched_getaffinity(0)))
+    O.lapack_set_threads(threads)
+    n = args.cpu_n
+    A = gen_spd(torch, n, args.seed).cpu().numpy()          # symmetric: layout-free
+    B = gen_uniform(torch, n, args.seed + 1).cpu().numpy().T  # column-major view
+    A = np.asfortranarray(A)
+    B = np.asfortranarray(B)
+    # warm the library (threads, buffers) on a small case
+    O.spotrf(np.asfortranarray(A[:1024, :1024]))
+    t0 = time.perf_counter()
+    info = O.spotrf(A)
+    t1 = time.perf_counter()
+    ipiv, info2 = O.sgetrf(B)
+    t2 = time.perf_counter()
+    flops = n ** 3 / 3.0 + 2.0 * n ** 3 / 3.0
+    return {
+        "value": flops / (t2 - t0) / 1e12,
+        "unit": "TFLOP/s",
+        "cores": threads,
+        "kind": "port",
+        "sample": (f"LAPACK spotrf + sgetrf via {os.path.basename(path)} ({config.strip()}), "
+                   f"n={n} FP32 of the same input families (seeded on the GPU, copied to host), "
+                   f"{threads} threads; spotrf {t1 - t0:.2f} s, sgetrf {t2 - t1:.2f} s; "
+                   f"info {info}/{info2}"),
+        "potrf_tflops": n ** 3 / 3.0 / (t1 - t0) / 1e12,
+        "getrf_tflops": 2.0 * n ** 3 / 3.0 / (t2 - t1) / 1e12,
+    }
+
+
+def main():
+    args = parse()
+    world = int(os.environ.get("WORLD_SIZE", "1"))
+    if args.gpus != 1 or world != 1:
+        print("bench.py: this path is measured on one GPU only; "
+              f"--gpus {args.gpus} (WORLD_SIZE={world}) is not supported. "
+              "A multi-GPU layout is described in DESIGN.md but not built.",
+              file=sys.stderr)
+        sys.exit(2)
+
+    import numpy as np
+    import torch
+    from hygondoubleblockedfactorizations_amd import _lib
+
+    n = args.n
+    routines = [r.strip() for r in args.routines.split(",") if r.strip()]
+    pnb, pb, gnb, gb, tb = block_sizes(args, n)
+    lib = _lib.load()
+    stream = torch.cuda.current_stream()
+    h = _lib.Handle(stream=stream)
+    h.reserve(n)
+    P = _lib.ptr
+
+    log(f"n={n} potrf nb/b={pnb}/{pb} getrf nb/b={gnb}/{gb} trsm_block={tb}")
+    t_gen = time.perf_counter()
+    inputs = {}
+    if "potrf" in routines:
+        inputs["potrf"] = gen_spd(torch, n, args.seed)
+    if "getrf" in routines:
+        inputs["getrf"] = gen_uniform(torch, n, args.seed + 1)
+    torch.cuda.synchronize()
+    log(f"inputs generated in {time.perf_counter() - t_gen:.1f} s")
+
+    work = torch.empty(n, n, dtype=torch.float32, device="cuda")
+    info = torch.zeros(1, dtype=torch.int32, device="cuda")
+    ipiv = torch.zeros(n, dtype=torch.int32, device="cuda")
+
+    def run(routine):
+        if routine == "potrf":
+            lib.double_blocked_potrf(h.ptr, P(work), n, n, pnb, pb, P(info))
+        else:
+            lib.double_blocked_getrf(h.ptr, P(work), n, n, gnb, gb, 1, tb, P(ipiv), P(info))
+
+    times = {r: [] for r in routines}
+    stages = {}
+    ev0 = torch.cuda.Event(enable_timing=True)
+    ev1 = torch.cuda.Event(enable_timing=True)
+    for r in routines:
+        for i in range(args.warmup):
+            work.copy_(inputs[r])
+            torch.cuda.synchronize()
+            run(r)
+            torch.cuda.synchronize()
+            log(f"warmup {r} {i} done")
+    if not args.no_profile:
+        _lib.profile_begin(h)
+    for i in range(args.steps):
+        for r in routines:
+            work.copy_(inputs[r])
+            torch.cuda.synchronize()
+            ev0.record(stream)
+            run(r)
+            ev1.record(stream)
+            torch.cuda.synchronize()
+            times[r].append(ev0.elapsed_time(ev1))
+            log(f"step {i} {r}: {times[r][-1]:.2f} ms")
+    if not args.no_profile:
+        stages = _lib.profile_end(h)
+    st = h.last_error()
+    if st != 0:
+        log(f"library reported status {st}")
+        sys.exit(1)
+    if int(info[0]) != 0:
+        log(f"info = {int(info[0])}")
+
+    flops = {"potrf": n ** 3 / 3.0, "getrf": 2.0 * n ** 3 / 3.0}
+    per = {}
+    for r in routines:
+        ms = float(np.mean(times[r]))
+        per[r] = {"ms": ms, "tflops": flops[r] / (ms * 1e-3) / 1e12,
+                  "frac_fp32_peak": flops[r] / (ms * 1e-3) / 1e12 / FP32_PEAK_TFLOPS,
+                  "nb": pnb if r == "potrf" else gnb, "b": pb if r == "potrf" else gb}
+
+    if not args.no_verify:
+        for r in routines:
+            work.copy_(inputs[r])
+            torch.cuda.synchronize()
+            run(r)
+            torch.cuda.synchronize()
+            per[r]["info"] = int(info[0])
+            if r == "potrf":
+                res = potrf_residual_gpu(torch, inputs[r], work)
+            else:
+                res = getrf_residual_gpu(torch, inputs[r], work, ipiv)
+            per[r]["residual_fp64"] = res
+            per[r]["residual_scaled"] = res / (n * float(np.finfo(np.float32).eps))
+            log(f"{r} residual {res:.3e}")
+
+    total_ms = sum(per[r]["ms"] for r in routines)
+    total_flops = sum(flops[r] for r in routines)
+    value = total_flops / (total_ms * 1e-3) / 1e12
+
+    roofline = None
+    g = stages.get("gemm")
+    if g and g["ms"] > 0:
+        achieved = g["flops"] / (g["ms"] * 1e-3) / 1e12
+        roofline = {"bound": "mfma", "achieved": achieved, "peak": FP32_PEAK_TFLOPS,
+                    "unit": "TFLOP/s", "frac": achieved / FP32_PEAK_TFLOPS, "traffic": None,
+                    "kernel": "mifact::gemm_kernel<float> (MFMA 32x32x2 f32)",
+                    "launches": g["calls"],
+                    "avg_launch_ms": g["ms"] / max(g["calls"], 1),
+                    "flops_per_launch": g["flops"] / max(g["calls"], 1)}
+    stage_out = {k: {"calls": v["calls"], "ms": v["ms"] / max(args.steps, 1),
+                     "tflops": (v["flops"] / (v["ms"] * 1e-3) / 1e12) if v["ms"] > 0 else None,
+                     "gbs": (v["bytes"] / (v["ms"] * 1e-3) / 1e9) if v["ms"] > 0 else None}
+                 for k, v in stages.items()}
+
+    cpu = None
+    if not args.no_cpu_baseline:
+        try:
+            cpu = cpu_baseline(torch, args)
+        except Exception as e:  # report, never hide the GPU line
+            log(f"cpu baseline failed: {e}")
+
+    line = {
+        "metric": METRIC,
+        "value": value,
+        "unit": "TFLOP/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": total_ms,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": f"double-blocked POTRF + GETRF (partial pivoting), FP32, "
+                               f"n={n}, column-major, lda=n",
+                   "n": n, "potrf_nb": pnb, "potrf_b": pb, "getrf_nb": gnb, "getrf_b": gb,
+                   "getrf_outer_trsm": "iterative", "trsm_block": tb},
+        "routines": per,
+        "roofline": roofline,
+        "cpu_baseline": cpu,
+        "stages_per_step": stage_out,
+    }
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -30,7 +30,8 @@ template <> struct Mfma<float> {
     using acc_t = floatx16;
     static constexpr int BLK = 32;
     static constexpr int KSTEP = 2;
-    static constexpr int GROUPS = 4;  // groups of 4 consecutive output rows
+    static constexpr int GROUPS = 4;      // groups of 4 output rows per lane
+    static constexpr int ROW_STRIDE = 1;  // rows of a group are consecutive
     __device__ static int op_row(int lane) { return lane & 31; }
     __device__ static int op_k(int lane) { return lane >> 5; }
     __device__ static int out_row(int lane, int g) { return g * 8 + (lane >> 5) * 4; }
@@ -42,15 +43,16 @@ template <> struct Mfma<float> {
 };
 
 // v_mfma_f64_16x16x4_f64: A lane l holds A[l%16][l/16]; B lane l holds
-// B[l/16][l%16]; acc[r] holds D[4*(l/16) + r][l%16].
+// B[l/16][l%16]; acc[r] holds D[l/16 + 4*r][l%16] (rows interleaved by 4).
 template <> struct Mfma<double> {
     using acc_t = doublex4;
     static constexpr int BLK = 16;
     static constexpr int KSTEP = 4;
     static constexpr int GROUPS = 1;
+    static constexpr int ROW_STRIDE = 4;
     __device__ static int op_row(int lane) { return lane & 15; }
     __device__ static int op_k(int lane) { return lane >> 4; }
-    __device__ static int out_row(int lane, int) { return (lane >> 4) * 4; }
+    __device__ static int out_row(int lane, int) { return lane >> 4; }
     __device__ static int out_col(int lane) { return lane & 15; }
     __device__ static acc_t mma(double a, double b, acc_t c)
     {
@@ -302,18 +304,19 @@ __global__ __launch_bounds__(256) void gemm_kernel(GemmArgs<T> p)
             if (col >= N) continue;
 #pragma unroll
             for (int g = 0; g < MF::GROUPS; ++g) {
+                constexpr int RS = MF::ROW_STRIDE;
                 const int row = m0 + wm * WM + i * BLK + MF::out_row(lane, g);
                 T* c = C + off(row, col, p.ldc);
-                const bool full = (row + 4 <= M) && (SHAPE != kShapeLower || row >= col);
+                const bool full = (row + 3 * RS < M) && (SHAPE != kShapeLower || row >= col);
                 if (full) {
 #pragma unroll
-                    for (int e = 0; e < 4; ++e) c[e] += alpha * acc[i][j][g * 4 + e];
+                    for (int e = 0; e < 4; ++e) c[e * RS] += alpha * acc[i][j][g * 4 + e];
                 } else {
 #pragma unroll
                     for (int e = 0; e < 4; ++e) {
-                        const int r = row + e;
+                        const int r = row + e * RS;
                         if (r < M && (SHAPE != kShapeLower || r >= col)) {
-                            c[e] += alpha * acc[i][j][g * 4 + e];
+                            c[e * RS] += alpha * acc[i][j][g * 4 + e];
                         }
                     }
                 }
