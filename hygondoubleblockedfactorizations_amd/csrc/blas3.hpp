@@ -14,40 +14,79 @@ inline int split_point(int w, int align)
     return w1;
 }
 
-// B(rows x cols) := B * L^{-T}, L lower non-unit (cols x cols).
+// Scratch for the inverses of the 64 x 64 diagonal blocks of a solve
+// (column-major, ld 64; the leaf starting at local column c sits at
+// inv + 64 c).  Sized for the workspace order.
+template <typename T>
+T* invdiag(mifact_handle h)
+{
+    return static_cast<T*>(workspace(h).invdiag);
+}
+
+// B(rows x cols) := B * L^{-T}, L lower non-unit (cols x cols), given the
+// inverses of L's 64 x 64 diagonal blocks at inv (leaf at local column c:
+// inv + 64 c).  Recursive halving at multiples of 64; each leaf is an
+// in-place MFMA multiply X = B inv(L_cc)^T.
+template <typename T>
+void trsm_rlt_inv(mifact_handle h, int rows, int cols, const T* L, long long ldl, T* B,
+                  long long ldb, const T* inv)
+{
+    if (rows <= 0 || cols <= 0) return;
+    if (cols <= 64) {
+        gemm_overwrite<T>(h, kOpNT, rows, cols, cols, T(1), B, ldb, inv, 64, B, ldb);
+        return;
+    }
+    const int c1 = split_point(cols, 64);
+    const int c2 = cols - c1;
+    trsm_rlt_inv<T>(h, rows, c1, L, ldl, B, ldb, inv);
+    gemm<T>(h, kOpNT, kShapeFull, rows, c2, c1, T(-1), B, ldb, at(L, ldl, c1, 0), ldl,
+            at(B, ldb, 0, c1), ldb);
+    trsm_rlt_inv<T>(h, rows, c2, at(L, ldl, c1, c1), ldl, at(B, ldb, 0, c1), ldb,
+                    inv + static_cast<size_t>(c1) * 64);
+}
+
+// B(rows x cols) := L^{-1} B, L lower (rows x rows; unit or not as the
+// inverses were formed), leaves multiply by inv(L_rr) in place on MFMA.
+template <typename T>
+void trsm_llu_inv(mifact_handle h, int rows, int cols, const T* L, long long ldl, T* B,
+                  long long ldb, const T* inv)
+{
+    if (rows <= 0 || cols <= 0) return;
+    if (rows <= 64) {
+        gemm_overwrite<T>(h, kOpNN, rows, cols, rows, T(1), inv, 64, B, ldb, B, ldb);
+        return;
+    }
+    const int r1 = split_point(rows, 64);
+    const int r2 = rows - r1;
+    trsm_llu_inv<T>(h, r1, cols, L, ldl, B, ldb, inv);
+    gemm<T>(h, kOpNN, kShapeFull, r2, cols, r1, T(-1), at(L, ldl, r1, 0), ldl, B, ldb,
+            at(B, ldb, r1, 0), ldb);
+    trsm_llu_inv<T>(h, r2, cols, at(L, ldl, r1, r1), ldl, at(B, ldb, r1, 0), ldb,
+                    inv + static_cast<size_t>(r1) * 64);
+}
+
+// B := B * L^{-T} for a standalone lower non-unit L (inverses formed here).
 template <typename T>
 void trsm_rlt(mifact_handle h, int rows, int cols, const T* L, long long ldl, T* B,
               long long ldb)
 {
     if (rows <= 0 || cols <= 0) return;
-    if (cols <= 64) {
-        trsm_rlt_base<T>(h, rows, cols, L, ldl, B, ldb);
-        return;
-    }
-    const int c1 = split_point(cols, 64);
-    const int c2 = cols - c1;
-    trsm_rlt<T>(h, rows, c1, L, ldl, B, ldb);
-    gemm<T>(h, kOpNT, kShapeFull, rows, c2, c1, T(-1), B, ldb, at(L, ldl, c1, 0), ldl,
-            at(B, ldb, 0, c1), ldb);
-    trsm_rlt<T>(h, rows, c2, at(L, ldl, c1, c1), ldl, at(B, ldb, 0, c1), ldb);
+    if (!ensure_workspace(h, cols)) return;
+    T* inv = invdiag<T>(h);
+    trtri_blocks<T>(h, cols, L, ldl, inv, false);
+    trsm_rlt_inv<T>(h, rows, cols, L, ldl, B, ldb, inv);
 }
 
-// B(rows x cols) := L^{-1} B, L unit lower (rows x rows).
+// B := L^{-1} B for a standalone unit lower L (inverses formed here).
 template <typename T>
 void trsm_llu(mifact_handle h, int rows, int cols, const T* L, long long ldl, T* B,
               long long ldb)
 {
     if (rows <= 0 || cols <= 0) return;
-    if (rows <= 64) {
-        trsm_llu_base<T>(h, rows, cols, L, ldl, B, ldb);
-        return;
-    }
-    const int r1 = split_point(rows, 64);
-    const int r2 = rows - r1;
-    trsm_llu<T>(h, r1, cols, L, ldl, B, ldb);
-    gemm<T>(h, kOpNN, kShapeFull, r2, cols, r1, T(-1), at(L, ldl, r1, 0), ldl, B, ldb,
-            at(B, ldb, r1, 0), ldb);
-    trsm_llu<T>(h, r2, cols, at(L, ldl, r1, r1), ldl, at(B, ldb, r1, 0), ldb);
+    if (!ensure_workspace(h, rows)) return;
+    T* inv = invdiag<T>(h);
+    trtri_blocks<T>(h, rows, L, ldl, inv, true);
+    trsm_llu_inv<T>(h, rows, cols, L, ldl, B, ldb, inv);
 }
 
 // Row-blocked variant: diagonal solves of `block` rows, each followed by a
@@ -73,22 +112,25 @@ void trsm_llu_iterative(mifact_handle h, int rows, int cols, const T* L, long lo
 }
 
 // In-place lower Cholesky of an s x s diagonal block (any s): recursive
-// halving down to 64-wide leaves.  info: first failing global column.
+// halving down to 64-wide one-wave leaves, which also leave the inverses of
+// their blocks at inv (local column c at inv + 64 c) for the panel solve.
+// info: first failing global column.
 template <typename T>
-void potrf_diag(mifact_handle h, int s, T* A, long long lda, int* info, int col_offset)
+void potrf_diag(mifact_handle h, int s, T* A, long long lda, int* info, int col_offset, T* inv)
 {
     if (s <= 0) return;
     if (s <= 64) {
-        potf2_base<T>(h, s, A, lda, info, col_offset);
+        potf2_base<T>(h, s, A, lda, inv, info, col_offset);
         return;
     }
     const int s1 = split_point(s, 64);
     const int s2 = s - s1;
-    potrf_diag<T>(h, s1, A, lda, info, col_offset);
-    trsm_rlt<T>(h, s2, s1, A, lda, at(A, lda, s1, 0), lda);
+    potrf_diag<T>(h, s1, A, lda, info, col_offset, inv);
+    trsm_rlt_inv<T>(h, s2, s1, A, lda, at(A, lda, s1, 0), lda, inv);
     gemm<T>(h, kOpNT, kShapeLower, s2, s2, s1, T(-1), at(A, lda, s1, 0), lda,
             at(A, lda, s1, 0), lda, at(A, lda, s1, s1), lda);
-    potrf_diag<T>(h, s2, at(A, lda, s1, s1), lda, info, col_offset + s1);
+    potrf_diag<T>(h, s2, at(A, lda, s1, s1), lda, info, col_offset + s1,
+                  inv + static_cast<size_t>(s1) * 64);
 }
 
 // Recursive partial-pivot LU of rows [c0, n) x columns [c0, c0 + w) of A.

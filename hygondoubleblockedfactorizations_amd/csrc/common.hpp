@@ -51,6 +51,8 @@ struct mifact_handle_s {
     // issued so far (each launch waits for targets above this base).
     unsigned long long* barrier = nullptr;
     unsigned long long barrier_base = 0;
+    // Epoch of the next GETF2 leaf column (exchange records carry it).
+    unsigned int leaf_epoch = 1;
     // Device status word written by kernels whose bounded waits expired.
     int* kernel_status = nullptr;
 
@@ -117,6 +119,8 @@ struct Workspace {
     int* inv = nullptr;          // kPermLevels x n ints: snapshots of orig^-1
     int* moves = nullptr;        // 4 x n ints: move lists (src 2n | dst 2n)
     float* exchange = nullptr;   // cooperative panel exchange records
+    void* invdiag = nullptr;     // n x 64 elements (double-sized): inverses of
+                                 // the 64 x 64 diagonal blocks of a solve
 };
 
 constexpr int kPermLevels = 24;             // panel/outer + recursion levels

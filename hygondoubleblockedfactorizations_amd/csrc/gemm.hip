@@ -122,7 +122,7 @@ __device__ inline void tile_lower(int bid, int tiles_m, int tiles_n, int& tm, in
 }
 
 template <typename T, int BM, int BN, int BK, int WAVES_M, int WAVES_N, int OP,
-          int SHAPE, bool VEC>
+          int SHAPE, bool VEC, bool BETA0>
 __global__ __launch_bounds__(256) void gemm_kernel(GemmArgs<T> p)
 {
     using MF = Mfma<T>;
@@ -160,8 +160,10 @@ __global__ __launch_bounds__(256) void gemm_kernel(GemmArgs<T> p)
     const int wn = wave / WAVES_M;
 
     const int M = p.M, N = p.N, K = p.K;
-    const T* __restrict__ A = p.A;
-    const T* __restrict__ B = p.B;
+    // No __restrict__: in overwrite mode C may alias A or B (each workgroup
+    // reads every input element of its tile before the epilogue writes).
+    const T* A = p.A;
+    const T* B = p.B;
 
     vec_t ra[A_VECS];
     vec_t rb[B_VECS];
@@ -295,7 +297,7 @@ __global__ __launch_bounds__(256) void gemm_kernel(GemmArgs<T> p)
 
     // Epilogue: C += alpha * acc, in groups of 4 consecutive rows.
     const T alpha = p.alpha;
-    T* __restrict__ C = p.C;
+    T* C = p.C;
 #pragma unroll
     for (int i = 0; i < FM; ++i) {
 #pragma unroll
@@ -310,13 +312,17 @@ __global__ __launch_bounds__(256) void gemm_kernel(GemmArgs<T> p)
                 const bool full = (row + 3 * RS < M) && (SHAPE != kShapeLower || row >= col);
                 if (full) {
 #pragma unroll
-                    for (int e = 0; e < 4; ++e) c[e * RS] += alpha * acc[i][j][g * 4 + e];
+                    for (int e = 0; e < 4; ++e) {
+                        if (BETA0) c[e * RS] = alpha * acc[i][j][g * 4 + e];
+                        else c[e * RS] += alpha * acc[i][j][g * 4 + e];
+                    }
                 } else {
 #pragma unroll
                     for (int e = 0; e < 4; ++e) {
                         const int r = row + e * RS;
                         if (r < M && (SHAPE != kShapeLower || r >= col)) {
-                            c[e * RS] += alpha * acc[i][j][g * 4 + e];
+                            if (BETA0) c[e * RS] = alpha * acc[i][j][g * 4 + e];
+                            else c[e * RS] += alpha * acc[i][j][g * 4 + e];
                         }
                     }
                 }
@@ -326,7 +332,7 @@ __global__ __launch_bounds__(256) void gemm_kernel(GemmArgs<T> p)
 }
 
 template <typename T, int BM, int BN, int BK, int WAVES_M, int WAVES_N>
-static void launch_cfg(mifact_handle h, GemmOp op, GemmShape shape, bool vec,
+static void launch_cfg(mifact_handle h, GemmOp op, GemmShape shape, bool vec, bool beta0,
                        const GemmArgs<T>& a0)
 {
     GemmArgs<T> a = a0;
@@ -347,10 +353,21 @@ static void launch_cfg(mifact_handle h, GemmOp op, GemmShape shape, bool vec,
                      sizeof(T) * ((static_cast<double>(a.M) + a.N) * a.K + 2.0 * mnk_c));
     const dim3 grid(static_cast<unsigned>(blocks));
     const dim3 block(256);
-#define MIFACT_GEMM_LAUNCH(OPV, SHV, VV)                                          \
-    hipLaunchKernelGGL((gemm_kernel<T, BM, BN, BK, WAVES_M, WAVES_N, OPV, SHV, VV>), \
+#define MIFACT_GEMM_LAUNCH(OPV, SHV, VV)                                                    \
+    hipLaunchKernelGGL((gemm_kernel<T, BM, BN, BK, WAVES_M, WAVES_N, OPV, SHV, VV, false>),    \
                        grid, block, 0, h->stream, a)
-    if (op == kOpNT) {
+#define MIFACT_GEMM_LAUNCH0(OPV, VV)                                                         \
+    hipLaunchKernelGGL((gemm_kernel<T, BM, BN, BK, WAVES_M, WAVES_N, OPV, kShapeFull, VV, true>), \
+                       grid, block, 0, h->stream, a)
+    if (beta0) {
+        if (op == kOpNT) {
+            if (vec) MIFACT_GEMM_LAUNCH0(kOpNT, true);
+            else MIFACT_GEMM_LAUNCH0(kOpNT, false);
+        } else {
+            if (vec) MIFACT_GEMM_LAUNCH0(kOpNN, true);
+            else MIFACT_GEMM_LAUNCH0(kOpNN, false);
+        }
+    } else if (op == kOpNT) {
         if (shape == kShapeFull) {
             if (vec) MIFACT_GEMM_LAUNCH(kOpNT, kShapeFull, true);
             else MIFACT_GEMM_LAUNCH(kOpNT, kShapeFull, false);
@@ -368,6 +385,7 @@ static void launch_cfg(mifact_handle h, GemmOp op, GemmShape shape, bool vec,
         }
     }
 #undef MIFACT_GEMM_LAUNCH
+#undef MIFACT_GEMM_LAUNCH0
     MIFACT_LAUNCHED(h);
 }
 
@@ -379,9 +397,9 @@ static bool aligned_for_vectors(const void* p, long long ld)
 }
 
 template <typename T>
-void gemm(mifact_handle h, GemmOp op, GemmShape shape, int M, int N, int K,
-          T alpha, const T* A, long long lda, const T* B, long long ldb, T* C,
-          long long ldc)
+static void dispatch(mifact_handle h, GemmOp op, GemmShape shape, bool beta0, int M, int N, int K,
+                     T alpha, const T* A, long long lda, const T* B, long long ldb, T* C,
+                     long long ldc)
 {
     if (M <= 0 || N <= 0 || K <= 0) return;
     if (shape == kShapeLower && M < N) {
@@ -393,16 +411,28 @@ void gemm(mifact_handle h, GemmOp op, GemmShape shape, int M, int N, int K,
     const bool vec = aligned_for_vectors<T>(A, lda) && aligned_for_vectors<T>(B, ldb);
     if constexpr (sizeof(T) == 4) {
         const long long big = static_cast<long long>(ceil_div(M, 128)) * ceil_div(N, 128);
-        if (N > 64 && M > 64 && big >= 2LL * h->num_cu) {
-            launch_cfg<T, 128, 128, 16, 2, 2>(h, op, shape, vec, a);
-        } else if (N <= 64 || shape == kShapeLower) {
-            launch_cfg<T, 64, 64, 16, 2, 2>(h, op, shape, vec, a);
+        if (!beta0 && N > 64 && M > 64 && big >= 2LL * h->num_cu) {
+            launch_cfg<T, 128, 128, 16, 2, 2>(h, op, shape, vec, beta0, a);
         } else {
-            launch_cfg<T, 64, 64, 16, 2, 2>(h, op, shape, vec, a);
+            launch_cfg<T, 64, 64, 16, 2, 2>(h, op, shape, vec, beta0, a);
         }
     } else {
-        launch_cfg<T, 64, 64, 16, 2, 2>(h, op, shape, vec, a);
+        launch_cfg<T, 64, 64, 16, 2, 2>(h, op, shape, vec, beta0, a);
     }
+}
+
+template <typename T>
+void gemm(mifact_handle h, GemmOp op, GemmShape shape, int M, int N, int K, T alpha, const T* A,
+          long long lda, const T* B, long long ldb, T* C, long long ldc)
+{
+    dispatch<T>(h, op, shape, false, M, N, K, alpha, A, lda, B, ldb, C, ldc);
+}
+
+template <typename T>
+void gemm_overwrite(mifact_handle h, GemmOp op, int M, int N, int K, T alpha, const T* A,
+                    long long lda, const T* B, long long ldb, T* C, long long ldc)
+{
+    dispatch<T>(h, op, kShapeFull, true, M, N, K, alpha, A, lda, B, ldb, C, ldc);
 }
 
 template void gemm<float>(mifact_handle, GemmOp, GemmShape, int, int, int, float,
@@ -411,5 +441,9 @@ template void gemm<float>(mifact_handle, GemmOp, GemmShape, int, int, int, float
 template void gemm<double>(mifact_handle, GemmOp, GemmShape, int, int, int, double,
                            const double*, long long, const double*, long long,
                            double*, long long);
+template void gemm_overwrite<float>(mifact_handle, GemmOp, int, int, int, float, const float*,
+                                    long long, const float*, long long, float*, long long);
+template void gemm_overwrite<double>(mifact_handle, GemmOp, int, int, int, double, const double*,
+                                     long long, const double*, long long, double*, long long);
 
 }  // namespace mifact

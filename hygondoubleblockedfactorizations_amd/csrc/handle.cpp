@@ -43,10 +43,13 @@ void report_status(mifact_handle h, mifact_status s, const char* what)
 
 static size_t int_slots(size_t n) { return (1 + kPermLevels + 4) * n; }
 
+static size_t invdiag_bytes(size_t n) { return (n + 64) * 64 * sizeof(double); }
+
 static size_t workspace_bytes(int n)
 {
     const size_t ints = int_slots(static_cast<size_t>(n));
-    return ((ints * sizeof(int) + 255) & ~static_cast<size_t>(255)) + kExchangeBytes;
+    return ((ints * sizeof(int) + 255) & ~static_cast<size_t>(255)) + kExchangeBytes
+         + invdiag_bytes(static_cast<size_t>(n));
 }
 
 bool ensure_workspace(mifact_handle h, int n)
@@ -73,6 +76,13 @@ bool ensure_workspace(mifact_handle h, int n)
     h->ws = p;
     h->ws_bytes = bytes;
     h->ws_n = want;
+    // Exchange records start with epoch 0, which no leaf column uses.
+    Workspace w = workspace(h);
+    if (hipMemset(w.exchange, 0, kExchangeBytes) != hipSuccess) {
+        report_status(h, mifact_status_hip_error, "workspace initialization failed");
+        return false;
+    }
+    h->leaf_epoch = 1;
     return true;
 }
 
@@ -86,6 +96,7 @@ Workspace workspace(mifact_handle h)
     w.moves = w.inv + static_cast<size_t>(kPermLevels) * n;
     const size_t off_bytes = (int_slots(n) * sizeof(int) + 255) & ~static_cast<size_t>(255);
     w.exchange = reinterpret_cast<float*>(base + off_bytes);
+    w.invdiag = base + off_bytes + kExchangeBytes;
     return w;
 }
 

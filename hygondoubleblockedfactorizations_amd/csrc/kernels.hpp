@@ -20,22 +20,25 @@ void gemm(mifact_handle h, GemmOp op, GemmShape shape, int M, int N, int K,
           T alpha, const T* A, long long lda, const T* B, long long ldb, T* C,
           long long ldc);
 
-// X * L^T = B for X (rows x s, s <= 64), L lower non-unit; B overwritten.
+// C(M x N) = alpha * A * op(B) (no accumulation into C; C may alias A or B
+// when every workgroup's tile reads only the rows/columns it writes, as in
+// the triangular-solve leaves).
 template <typename T>
-void trsm_rlt_base(mifact_handle h, int rows, int s, const T* L, long long ldl,
-                   T* B, long long ldb);
+void gemm_overwrite(mifact_handle h, GemmOp op, int M, int N, int K, T alpha, const T* A,
+                    long long lda, const T* B, long long ldb, T* C, long long ldc);
 
-// L * X = B for X (s x cols, s <= 64), L lower unit; B overwritten.
+// Unblocked lower Cholesky of an s x s block (s <= 64) in one wave.  On a
+// non-positive pivot at local column j: *info = col_offset + j + 1 if *info
+// was 0 (first failure in launch order wins).  When inv != null, inv(L) is
+// written column-major with ld 64.
 template <typename T>
-void trsm_llu_base(mifact_handle h, int s, int cols, const T* L, long long ldl,
-                   T* B, long long ldb);
-
-// Unblocked lower Cholesky of an s x s block (s <= 64) in one workgroup.
-// On a non-positive pivot at local column j: *info = col_offset + j + 1 if
-// *info was 0 (first failure in launch order wins).
-template <typename T>
-void potf2_base(mifact_handle h, int s, T* A, long long lda, int* info,
+void potf2_base(mifact_handle h, int s, T* A, long long lda, T* inv, int* info,
                 int col_offset);
+
+// Inverses of the 64 x 64 diagonal blocks of the n x n lower triangle of L
+// (unit diagonal when unit): block b at inv + 4096 * b, ld 64.
+template <typename T>
+void trtri_blocks(mifact_handle h, int n, const T* L, long long ldl, T* inv, bool unit);
 
 // Cooperative partial-pivot LU of a leaf panel: rows [c0, n) x columns
 // [c0, c0 + w) of the full matrix (w <= 32).  Writes ipiv[c0 + j] (global,

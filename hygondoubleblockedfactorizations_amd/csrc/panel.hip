@@ -1,183 +1,99 @@
-// Panel kernels for gfx950: diagonal POTF2, register-blocked triangular
-// solves, wave-reduction IAMAX.
+// Panel kernels for gfx950: diagonal POTF2 (with the inverse of its block),
+// batched inverses of 64 x 64 diagonal blocks, wave-reduction IAMAX.
 //
 // These are the leaves of the recursive diagonal factorization and of the
-// recursive triangular solves; everything above the leaves is MFMA GEMM.
+// recursive triangular solves.  A triangular-solve leaf multiplies by the
+// inverse of its 64 x 64 diagonal block on MFMA (as rocBLAS' TRSM does with
+// its inverted diagonal blocks); everything above the leaves is MFMA GEMM.
 #include <hip/hip_runtime.h>
 
 #include "kernels.hpp"
 
 namespace mifact {
 
-constexpr int kLeaf = 64;  // leaf order of POTF2 / TRSM kernels
+constexpr int kLeaf = 64;  // leaf order of POTF2 / triangular inverse kernels
 
-// ------------------------------------------------------------------ TRSM RLT
-// X L^T = B, one thread per row of B: x[] lives in registers, L in LDS (all
-// lanes read the same L entry: broadcast).  Coalesced: a column of B is
-// contiguous and consecutive threads own consecutive rows.
-template <typename T>
-__global__ __launch_bounds__(64) void trsm_rlt_kernel(int rows, int s, const T* __restrict__ L,
-                                                      long long ldl, T* __restrict__ B,
-                                                      long long ldb)
+// Row i of M = inv(L) for a lower-triangular s x s block held in LDS
+// (sL[k][j] = L(k, j)), computed by thread i from the last column to the
+// first: M(i,i) = 1/L(i,i) (1 when unit), M(i,j) = -sum_{k>j} M(i,k) L(k,j)
+// / L(j,j).  Every LDS read is a broadcast (same address on all lanes).
+template <typename T, bool UNIT>
+__device__ __attribute__((always_inline)) inline void inverse_row(int s, const T (*sL)[kLeaf + 1],
+                                                                 T (&m)[kLeaf])
 {
-    __shared__ T sL[kLeaf][kLeaf + 1];
-    __shared__ T sD[kLeaf];
-    const int tid = threadIdx.x;
-    for (int idx = tid; idx < s * s; idx += 64) {
-        const int i = idx % s, j = idx / s;
-        if (i >= j) sL[i][j] = L[off(i, j, ldl)];
-    }
-    __syncthreads();
-    if (tid < s) sD[tid] = T(1) / sL[tid][tid];
-    __syncthreads();
-    const int row = blockIdx.x * 64 + tid;
-    if (row >= rows) return;
-    T x[kLeaf];
+    const int i = threadIdx.x;
 #pragma unroll
-    for (int j = 0; j < kLeaf; ++j) x[j] = (j < s) ? B[off(row, j, ldb)] : T(0);
-#pragma unroll
-    for (int j = 0; j < kLeaf; ++j) {
+    for (int j = kLeaf - 1; j >= 0; --j) {
         if (j < s) {
-            T acc = x[j];
+            T acc = T(0);
 #pragma unroll
-            for (int k = 0; k < j; ++k) acc -= x[k] * sL[j][k];
-            x[j] = acc * sD[j];
+            for (int k = j + 1; k < kLeaf; ++k) acc += m[k] * sL[k][j];
+            const T dj = UNIT ? T(1) : T(1) / sL[j][j];
+            m[j] = (i == j) ? dj : (i > j ? -acc * dj : T(0));
+        } else {
+            m[j] = T(0);
         }
     }
-#pragma unroll
-    for (int j = 0; j < kLeaf; ++j)
-        if (j < s) B[off(row, j, ldb)] = x[j];
-}
-
-template <typename T>
-void trsm_rlt_base(mifact_handle h, int rows, int s, const T* L, long long ldl,
-                   T* B, long long ldb)
-{
-    if (rows <= 0 || s <= 0) return;
-    if (s > kLeaf) {
-        report_status(h, mifact_status_invalid_value, "trsm_rlt_base: s > 64");
-        return;
-    }
-    StageScope scope(h, kStageTrsmLeaf, static_cast<double>(rows) * s * s,
-                     sizeof(T) * (2.0 * rows * s + 0.5 * s * s));
-    hipLaunchKernelGGL(trsm_rlt_kernel<T>, dim3(ceil_div(rows, 64)), dim3(64), 0,
-                       h->stream, rows, s, L, ldl, B, ldb);
-    MIFACT_LAUNCHED(h);
-}
-
-// ------------------------------------------------------------------ TRSM LLU
-// L X = B (L unit lower), one thread per column of B; the B tile is staged
-// through LDS so the global reads and writes are coalesced along columns.
-template <typename T>
-__global__ __launch_bounds__(64) void trsm_llu_kernel(int s, int cols, const T* __restrict__ L,
-                                                      long long ldl, T* __restrict__ B,
-                                                      long long ldb)
-{
-    __shared__ T sL[kLeaf][kLeaf + 1];
-    __shared__ T sB[kLeaf][64 + 1];
-    const int tid = threadIdx.x;
-    const int c0 = blockIdx.x * 64;
-    for (int idx = tid; idx < s * s; idx += 64) {
-        const int i = idx % s, j = idx / s;
-        if (i > j) sL[i][j] = L[off(i, j, ldl)];
-    }
-    const int nc = (cols - c0) < 64 ? (cols - c0) : 64;
-    for (int c = 0; c < nc; ++c)
-        for (int r = tid; r < s; r += 64) sB[r][c] = B[off(r, c0 + c, ldb)];
-    __syncthreads();
-    if (tid < nc) {
-        T x[kLeaf];
-#pragma unroll
-        for (int i = 0; i < kLeaf; ++i) x[i] = (i < s) ? sB[i][tid] : T(0);
-#pragma unroll
-        for (int i = 0; i < kLeaf; ++i) {
-            if (i < s) {
-                T acc = x[i];
-#pragma unroll
-                for (int k = 0; k < i; ++k) acc -= sL[i][k] * x[k];
-                x[i] = acc;
-            }
-        }
-#pragma unroll
-        for (int i = 0; i < kLeaf; ++i)
-            if (i < s) sB[i][tid] = x[i];
-    }
-    __syncthreads();
-    for (int c = 0; c < nc; ++c)
-        for (int r = tid; r < s; r += 64) B[off(r, c0 + c, ldb)] = sB[r][c];
-}
-
-template <typename T>
-void trsm_llu_base(mifact_handle h, int s, int cols, const T* L, long long ldl,
-                   T* B, long long ldb)
-{
-    if (s <= 0 || cols <= 0) return;
-    if (s > kLeaf) {
-        report_status(h, mifact_status_invalid_value, "trsm_llu_base: s > 64");
-        return;
-    }
-    StageScope scope(h, kStageTrsmLeaf, static_cast<double>(cols) * s * s,
-                     sizeof(T) * (2.0 * cols * s + 0.5 * s * s));
-    hipLaunchKernelGGL(trsm_llu_kernel<T>, dim3(ceil_div(cols, 64)), dim3(64), 0,
-                       h->stream, s, cols, L, ldl, B, ldb);
-    MIFACT_LAUNCHED(h);
 }
 
 // ------------------------------------------------------------------ POTF2
-// Right-looking unblocked Cholesky of an s x s (s <= 64) lower block held in
-// LDS by one 256-thread workgroup.  Reciprocal scaling of each column, as
-// LAPACK xPOTF2 does.
+// Unblocked lower Cholesky of an s x s block (s <= 64) by one wave64: lane i
+// holds row i in registers; per column j the pivot is broadcast with a lane
+// shuffle, the scaled column goes through LDS and every lane updates its row
+// (right-looking; reciprocal scaling as LAPACK xPOTF2).  When inv is not
+// null the wave then writes inv(L) (column-major, ld 64): the panel TRSM
+// applies it with an MFMA multiply instead of a substitution.
 template <typename T>
-__global__ __launch_bounds__(256) void potf2_kernel(int s, T* __restrict__ A, long long lda,
-                                                    int* __restrict__ info, int col_offset)
+__global__ __launch_bounds__(64) void potf2_kernel(int s, T* __restrict__ A, long long lda,
+                                                   T* __restrict__ inv, int* __restrict__ info,
+                                                   int col_offset)
 {
-    __shared__ T sA[kLeaf][kLeaf + 1];
-    __shared__ int failed;
-    const int tid = threadIdx.x;
-    for (int idx = tid; idx < s * s; idx += 256) {
-        const int i = idx % s, j = idx / s;
-        if (i >= j) sA[i][j] = A[off(i, j, lda)];
-    }
-    if (tid == 0) failed = 0;
-    __syncthreads();
-    int jend = s;
-    for (int j = 0; j < s; ++j) {
-        if (tid == 0) {
-            const T d = sA[j][j];
+    __shared__ T col[kLeaf];
+    __shared__ T sL[kLeaf][kLeaf + 1];
+    const int i = threadIdx.x;
+    T x[kLeaf];
+#pragma unroll
+    for (int k = 0; k < kLeaf; ++k) x[k] = (i < s && k <= i) ? A[off(i, k, lda)] : T(0);
+    bool failed = false;
+#pragma unroll
+    for (int j = 0; j < kLeaf; ++j) {
+        if (!failed && j < s) {
+            const T d = __shfl(x[j], j);
             if (!(d > T(0))) {
-                failed = 1;
-                if (info != nullptr) atomicCAS(info, 0, col_offset + j + 1);
+                failed = true;
+                if (i == 0 && info != nullptr) atomicCAS(info, 0, col_offset + j + 1);
             } else {
-                sA[j][j] = sqrt(d);
+                const T ljj = sqrt(d);
+                const T r = T(1) / ljj;
+                if (i == j) {
+                    x[j] = ljj;
+                } else if (i > j) {
+                    x[j] *= r;
+                }
+                col[i] = x[j];
+                __syncthreads();
+#pragma unroll
+                for (int k = j + 1; k < kLeaf; ++k) x[k] -= x[j] * col[k];
+                __syncthreads();
             }
         }
-        __syncthreads();
-        if (failed) {
-            jend = j;
-            break;
-        }
-        const T r = T(1) / sA[j][j];
-        for (int i = j + 1 + tid; i < s; i += 256) sA[i][j] *= r;
-        __syncthreads();
-        // trailing lower update: pairs (i, k) with j < k <= i < s
-        const int t = s - j - 1;
-        for (int idx = tid; idx < t * t; idx += 256) {
-            const int i = j + 1 + idx % t;
-            const int k = j + 1 + idx / t;
-            if (i >= k) sA[i][k] -= sA[i][j] * sA[k][j];
-        }
-        __syncthreads();
     }
-    (void)jend;
+#pragma unroll
+    for (int k = 0; k < kLeaf; ++k)
+        if (i < s && k <= i) A[off(i, k, lda)] = x[k];
+    if (inv == nullptr || failed) return;
+#pragma unroll
+    for (int k = 0; k < kLeaf; ++k) sL[i][k] = x[k];
     __syncthreads();
-    for (int idx = tid; idx < s * s; idx += 256) {
-        const int i = idx % s, j = idx / s;
-        if (i >= j) A[off(i, j, lda)] = sA[i][j];
-    }
+    T m[kLeaf];
+    inverse_row<T, false>(s, sL, m);
+#pragma unroll
+    for (int j = 0; j < kLeaf; ++j)
+        if (i < s && j < s) inv[j * kLeaf + i] = m[j];
 }
 
 template <typename T>
-void potf2_base(mifact_handle h, int s, T* A, long long lda, int* info, int col_offset)
+void potf2_base(mifact_handle h, int s, T* A, long long lda, T* inv, int* info, int col_offset)
 {
     if (s <= 0) return;
     if (s > kLeaf) {
@@ -185,9 +101,52 @@ void potf2_base(mifact_handle h, int s, T* A, long long lda, int* info, int col_
         return;
     }
     StageScope scope(h, kStagePotf2, static_cast<double>(s) * s * s / 3.0,
-                     sizeof(T) * static_cast<double>(s) * s);
-    hipLaunchKernelGGL(potf2_kernel<T>, dim3(1), dim3(256), 0, h->stream, s, A, lda,
+                     sizeof(T) * static_cast<double>(s) * s * 1.5);
+    hipLaunchKernelGGL(potf2_kernel<T>, dim3(1), dim3(64), 0, h->stream, s, A, lda, inv,
                        info, col_offset);
+    MIFACT_LAUNCHED(h);
+}
+
+// ------------------------------------------------------------------ TRTRI
+// Inverses of the 64 x 64 diagonal blocks of an n x n lower-triangular L
+// (unit or non-unit), one wave per block; block b is written at
+// inv + 64*64*b (column-major, ld 64).
+template <typename T, bool UNIT>
+__global__ __launch_bounds__(64) void trtri_kernel(int n, const T* __restrict__ L, long long ldl,
+                                                   T* __restrict__ inv)
+{
+    __shared__ T sL[kLeaf][kLeaf + 1];
+    const int b = blockIdx.x;
+    const int r0 = b * kLeaf;
+    const int s = (n - r0) < kLeaf ? (n - r0) : kLeaf;
+    const int i = threadIdx.x;
+    const T* Lb = L + off(r0, r0, ldl);
+#pragma unroll 8
+    for (int k = 0; k < kLeaf; ++k)
+        sL[i][k] = (i < s && k < s && k <= i) ? Lb[off(i, k, ldl)] : T(0);
+    __syncthreads();
+    T m[kLeaf];
+    inverse_row<T, UNIT>(s, sL, m);
+    T* ib = inv + static_cast<size_t>(b) * kLeaf * kLeaf;
+#pragma unroll
+    for (int j = 0; j < kLeaf; ++j)
+        if (i < s && j < s) ib[j * kLeaf + i] = m[j];
+}
+
+template <typename T>
+void trtri_blocks(mifact_handle h, int n, const T* L, long long ldl, T* inv, bool unit)
+{
+    if (n <= 0) return;
+    const int nb = ceil_div(n, kLeaf);
+    StageScope scope(h, kStageTrsmLeaf, static_cast<double>(nb) * kLeaf * kLeaf * kLeaf / 3.0,
+                     sizeof(T) * static_cast<double>(nb) * kLeaf * kLeaf * 1.5);
+    if (unit) {
+        hipLaunchKernelGGL((trtri_kernel<T, true>), dim3(nb), dim3(64), 0, h->stream, n, L, ldl,
+                           inv);
+    } else {
+        hipLaunchKernelGGL((trtri_kernel<T, false>), dim3(nb), dim3(64), 0, h->stream, n, L, ldl,
+                           inv);
+    }
     MIFACT_LAUNCHED(h);
 }
 
@@ -241,12 +200,10 @@ void iamax(mifact_handle h, int n, const T* x, int incx, int* result)
     MIFACT_LAUNCHED(h);
 }
 
-template void trsm_rlt_base<float>(mifact_handle, int, int, const float*, long long, float*, long long);
-template void trsm_rlt_base<double>(mifact_handle, int, int, const double*, long long, double*, long long);
-template void trsm_llu_base<float>(mifact_handle, int, int, const float*, long long, float*, long long);
-template void trsm_llu_base<double>(mifact_handle, int, int, const double*, long long, double*, long long);
-template void potf2_base<float>(mifact_handle, int, float*, long long, int*, int);
-template void potf2_base<double>(mifact_handle, int, double*, long long, int*, int);
+template void potf2_base<float>(mifact_handle, int, float*, long long, float*, int*, int);
+template void potf2_base<double>(mifact_handle, int, double*, long long, double*, int*, int);
+template void trtri_blocks<float>(mifact_handle, int, const float*, long long, float*, bool);
+template void trtri_blocks<double>(mifact_handle, int, const double*, long long, double*, bool);
 template void iamax<float>(mifact_handle, int, const float*, int, int*);
 template void iamax<double>(mifact_handle, int, const double*, int, int*);
 

@@ -33,7 +33,7 @@
 namespace mifact {
 
 constexpr int kLeafW = 32;
-constexpr int kMaxG = 256;
+constexpr int kMaxG = 128;  // workgroups of a leaf (rows <= 128 * 256 * 4)
 
 template <typename T> __device__ inline T sfmin();
 template <> __device__ inline float sfmin<float>() { return FLT_MIN; }
@@ -48,6 +48,82 @@ __device__ inline void amax_pick(T& v, int& i, T ov, int oi)
     }
 }
 
+// DPP row/bank moves (lanes whose source is out of range keep their own
+// value, which is neutral for the idempotent arg-max combine).
+template <int CTRL>
+__device__ inline int dpp_move(int v)
+{
+    return __builtin_amdgcn_update_dpp(v, v, CTRL, 0xf, 0xf, false);
+}
+template <int CTRL>
+__device__ inline float dpp_move(float v)
+{
+    return __int_as_float(dpp_move<CTRL>(__float_as_int(v)));
+}
+template <int CTRL>
+__device__ inline double dpp_move(double v)
+{
+    const long long b = __double_as_longlong(v);
+    const int lo = dpp_move<CTRL>(static_cast<int>(b));
+    const int hi = dpp_move<CTRL>(static_cast<int>(b >> 32));
+    return __longlong_as_double((static_cast<long long>(hi) << 32)
+                                | static_cast<unsigned int>(lo));
+}
+
+template <int CTRL, typename T>
+__device__ inline void dpp_step(T& v, int& i)
+{
+    const T ov = dpp_move<CTRL>(v);
+    const int oi = dpp_move<CTRL>(i);
+    amax_pick(v, i, ov, oi);
+}
+
+__device__ inline float lane63(float v) { return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), 63)); }
+__device__ inline double lane63(double v)
+{
+    const long long b = __double_as_longlong(v);
+    const int lo = __builtin_amdgcn_readlane(static_cast<int>(b), 63);
+    const int hi = __builtin_amdgcn_readlane(static_cast<int>(b >> 32), 63);
+    return __longlong_as_double((static_cast<long long>(hi) << 32) | static_cast<unsigned int>(lo));
+}
+
+// Wave64 arg-max with DPP butterflies inside rows of 16 (quad_perm [1,0,3,2],
+// quad_perm [2,3,0,1], row_ror 4, row_ror 8) and row broadcasts (15, 31);
+// the full result lands in lane 63 and is returned wave-uniform.
+template <typename T>
+__device__ inline void wave_argmax(T& v, int& i)
+{
+    dpp_step<0xb1>(v, i);
+    dpp_step<0x4e>(v, i);
+    dpp_step<0x124>(v, i);
+    dpp_step<0x128>(v, i);
+    dpp_step<0x142>(v, i);
+    dpp_step<0x143>(v, i);
+    v = lane63(v);
+    i = __builtin_amdgcn_readlane(i, 63);
+}
+
+// Exchange records.  For column j every workgroup publishes one record
+// (its pivot candidate: row index, |value|, the whole candidate row and the
+// row's origin tag) and workgroup 0 also publishes row j.  A record is RW
+// 64-bit words {payload: 32 bits, epoch: 32 bits}, each written by a single
+// coherent (sc1) 64-bit store, so a reader that sees column j's epoch in a
+// word also sees that word's payload.  Readers poll the records themselves
+// until every word carries the epoch: no counter, no fences, one round trip
+// when the data is already there.  Records are double-buffered by column
+// parity: a workgroup can only overwrite a parity slot (column j + 2) after
+// it has read every record of column j + 1, which every workgroup publishes
+// only after it has finished reading column j.
+template <typename T>
+struct Rec {
+    static constexpr int PW = sizeof(T) / 4;           // payload words per T
+    static constexpr int IDX = 0;                       // candidate row
+    static constexpr int VAL = 1;                       // |value|
+    static constexpr int ROW = 1 + PW;                  // kLeafW values
+    static constexpr int TAG = ROW + PW * kLeafW;       // row origin
+    static constexpr int RW = TAG + 1;                  // words per record
+};
+
 template <typename T>
 struct LeafArgs {
     T* A;
@@ -59,38 +135,60 @@ struct LeafArgs {
     int* ipiv;
     int* orig;
     int* info;
-    T* xval;       // [2][kMaxG + 1]
-    int* xidx;     // [2][kMaxG + 1]
-    T* xrow;       // [2][kMaxG + 1][kLeafW]
-    int* xtag;     // [2][kMaxG + 1]
-    unsigned long long* bar;
-    unsigned long long bar_base;
+    unsigned long long* xrec;  // [2][kMaxG + 1][RW]
+    unsigned int epoch0;       // epoch of column j is epoch0 + j
     int* status;
     int G;
 };
 
-// Grid barrier over G workgroups.  Returns false when the bounded wait
-// expired (the kernel then exits; the host reports a timeout).
-__device__ inline bool grid_barrier(unsigned long long* bar, unsigned long long target,
-                                    int* status, int* s_abort)
+__device__ inline void xstore64(unsigned long long* p, unsigned int payload, unsigned int epoch)
 {
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        __threadfence();
-        atomicAdd(bar, 1ULL);
-        unsigned long long spins = 0;
-        while (__hip_atomic_load(bar, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < target) {
-            if (++spins > (1ULL << 24)) {
-                atomicExch(status, 1);
-                *s_abort = 1;
-                break;
-            }
-            __builtin_amdgcn_s_sleep(1);
-        }
-        __threadfence();
+    const unsigned long long v = (static_cast<unsigned long long>(epoch) << 32) | payload;
+    __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+__device__ inline unsigned long long xload64(const unsigned long long* p)
+{
+    return __hip_atomic_load(const_cast<unsigned long long*>(p), __ATOMIC_RELAXED,
+                             __HIP_MEMORY_SCOPE_AGENT);
+}
+
+__device__ inline void to_words(float v, unsigned int* w) { w[0] = __float_as_uint(v); }
+__device__ inline void to_words(double v, unsigned int* w)
+{
+    const unsigned long long b = static_cast<unsigned long long>(__double_as_longlong(v));
+    w[0] = static_cast<unsigned int>(b);
+    w[1] = static_cast<unsigned int>(b >> 32);
+}
+template <typename T> __device__ inline T from_words(const unsigned int* w);
+template <> __device__ inline float from_words<float>(const unsigned int* w)
+{
+    return __uint_as_float(w[0]);
+}
+template <> __device__ inline double from_words<double>(const unsigned int* w)
+{
+    const unsigned long long b = (static_cast<unsigned long long>(w[1]) << 32) | w[0];
+    return __longlong_as_double(static_cast<long long>(b));
+}
+
+// Writes one record (one thread).
+template <typename T>
+__device__ inline void publish(unsigned long long* rec, unsigned int epoch, int idx, T val,
+                               const T (&row)[kLeafW], int tag)
+{
+    using R = Rec<T>;
+    unsigned int w[R::PW];
+    xstore64(rec + R::IDX, static_cast<unsigned int>(idx), epoch);
+    to_words(val, w);
+#pragma unroll
+    for (int q = 0; q < R::PW; ++q) xstore64(rec + R::VAL + q, w[q], epoch);
+#pragma unroll
+    for (int c = 0; c < kLeafW; ++c) {
+        to_words(row[c], w);
+#pragma unroll
+        for (int q = 0; q < R::PW; ++q) xstore64(rec + R::ROW + R::PW * c + q, w[q], epoch);
     }
-    __syncthreads();
-    return *s_abort == 0;
+    xstore64(rec + R::TAG, static_cast<unsigned int>(tag), epoch);
 }
 
 // Shared state of one leaf workgroup.
@@ -98,27 +196,29 @@ template <typename T>
 struct LeafShared {
     T wv[4];
     int wi[4];
-    T prow[kLeafW];
-    T jrow[kLeafW];
-    int ptag, jtag, p;
+    unsigned int rec[kMaxG + 1][Rec<T>::RW];  // payloads of every record of the column
+    int p;
     int abort;
 };
 
 // Column J of the leaf (J is a compile-time constant so the register rows
-// x[r][J] stay in registers).  Returns false when the grid barrier timed out.
+// x[r][J] stay in registers).  Returns false when the bounded wait expired.
 template <typename T, int RPT, int J>
 __device__ __attribute__((always_inline)) inline bool leaf_column(
     const LeafArgs<T>& a, LeafShared<T>& sh, T (&x)[RPT][kLeafW], int (&tag)[RPT], int base)
 {
+    using R = Rec<T>;
     const int tid = threadIdx.x;
     const int g = blockIdx.x;
     const int G = a.G;
     const int m = a.m;
     const int w = a.w;
-    const int R = kMaxG + 1;
     constexpr int j = J;
-    const int par = j & 1;
-    // 1. local candidate
+    const unsigned int epoch = a.epoch0 + static_cast<unsigned int>(j);
+    unsigned long long* slots =
+        a.xrec + static_cast<size_t>(j & 1) * (kMaxG + 1) * R::RW;
+
+    // 1. local candidate: max |a(i,j)| over own rows i >= j, smallest row on ties
     T best = T(-1);
     int bi = 0x7fffffff;
 #pragma unroll
@@ -126,12 +226,7 @@ __device__ __attribute__((always_inline)) inline bool leaf_column(
         const int lrow = base + r * 256 + tid;
         if (lrow >= j && lrow < m) amax_pick(best, bi, fabs(x[r][j]), lrow);
     }
-#pragma unroll
-    for (int o = 32; o > 0; o >>= 1) {
-        const T ov = __shfl_xor(best, o);
-        const int oi = __shfl_xor(bi, o);
-        amax_pick(best, bi, ov, oi);
-    }
+    wave_argmax(best, bi);
     if ((tid & 63) == 0) {
         sh.wv[tid >> 6] = best;
         sh.wi[tid >> 6] = bi;
@@ -142,75 +237,70 @@ __device__ __attribute__((always_inline)) inline bool leaf_column(
 #pragma unroll
     for (int q = 1; q < 4; ++q) amax_pick(wb, wi, sh.wv[q], sh.wi[q]);
 
-    // 2. publish
-    const size_t rec = static_cast<size_t>(par) * R + g;
+    // 2. publish the candidate (and row j from workgroup 0)
+    if (wi == 0x7fffffff) {
+        if (tid == 0) {
+            T zero[kLeafW];
 #pragma unroll
-    for (int r = 0; r < RPT; ++r) {
-        const int lrow = base + r * 256 + tid;
-        if (lrow == wi) {
-            a.xval[rec] = wb;
-            a.xidx[rec] = wi;
+            for (int c = 0; c < kLeafW; ++c) zero[c] = T(0);
+            publish<T>(slots + static_cast<size_t>(g) * R::RW, epoch, 0x7fffffff, T(-1), zero, -1);
+        }
+    } else {
 #pragma unroll
-            for (int c = 0; c < kLeafW; ++c) a.xrow[rec * kLeafW + c] = x[r][c];
-            a.xtag[rec] = tag[r];
+        for (int r = 0; r < RPT; ++r) {
+            const int lrow = base + r * 256 + tid;
+            if (lrow == wi) publish<T>(slots + static_cast<size_t>(g) * R::RW, epoch, wi, wb, x[r], tag[r]);
         }
     }
-    if (wi == 0x7fffffff && tid == 0) {
-        a.xval[rec] = T(-1);
-        a.xidx[rec] = 0x7fffffff;
-    }
-    if (g == 0 && tid == j) {
-        const size_t jrec = static_cast<size_t>(par) * R + G;
+    if (g == 0 && tid == j) publish<T>(slots + static_cast<size_t>(G) * R::RW, epoch, j, T(0), x[0], tag[0]);
+
+    // 3. gather every record of this column into LDS (poll until all carry the epoch)
+    const int total = (G + 1) * R::RW;
+    constexpr int kPoll = ((kMaxG + 1) * R::RW + 255) / 256;
+    unsigned long long spins = 0;
+    while (true) {
+        // issue every load of this thread before consuming any
+        unsigned long long v[kPoll];
 #pragma unroll
-        for (int c = 0; c < kLeafW; ++c) a.xrow[jrec * kLeafW + c] = x[0][c];
-        a.xtag[jrec] = tag[0];
+        for (int q = 0; q < kPoll; ++q) {
+            const int e = tid + q * 256;
+            v[q] = (e < total) ? xload64(slots + e) : 0ULL;
+        }
+        int miss = 0;
+#pragma unroll
+        for (int q = 0; q < kPoll; ++q) {
+            const int e = tid + q * 256;
+            if (e < total) {
+                if (static_cast<unsigned int>(v[q] >> 32) != epoch) {
+                    miss = 1;
+                } else {
+                    (&sh.rec[0][0])[e] = static_cast<unsigned int>(v[q]);
+                }
+            }
+        }
+        if (!__syncthreads_or(miss)) break;
+        if (++spins > (1ULL << 18)) {
+            if (tid == 0) atomicExch(a.status, 1);
+            return false;
+        }
+        __builtin_amdgcn_s_sleep(1);
     }
 
-    // 3. grid barrier
-    if (!grid_barrier(a.bar, a.bar_base + static_cast<unsigned long long>(j + 1) * G, a.status,
-                      &sh.abort)) {
-        return false;
-    }
-
-    // 4. global pivot
-    best = T(-1);
-    bi = 0x7fffffff;
-    for (int t = tid; t < G; t += 256) {
-        const size_t rr = static_cast<size_t>(par) * R + t;
-        amax_pick(best, bi, a.xval[rr], a.xidx[rr]);
-    }
-#pragma unroll
-    for (int o = 32; o > 0; o >>= 1) {
-        const T ov = __shfl_xor(best, o);
-        const int oi = __shfl_xor(bi, o);
-        amax_pick(best, bi, ov, oi);
-    }
-    if ((tid & 63) == 0) {
-        sh.wv[tid >> 6] = best;
-        sh.wi[tid >> 6] = bi;
-    }
-    __syncthreads();
-    if (tid == 0) {
-        T gb = sh.wv[0];
-        int gi = sh.wi[0];
-        for (int q = 1; q < 4; ++q) amax_pick(gb, gi, sh.wv[q], sh.wi[q]);
-        sh.p = (gi == 0x7fffffff) ? j : gi;
+    // 4. global pivot: max over the G candidates, smallest row on ties
+    if (tid < 64) {
+        T gb = T(-1);
+        int gi = 0x7fffffff;
+        for (int t = tid; t < G; t += 64)
+            amax_pick(gb, gi, from_words<T>(&sh.rec[t][R::VAL]), static_cast<int>(sh.rec[t][R::IDX]));
+        wave_argmax(gb, gi);
+        if (tid == 0) sh.p = (gi == 0x7fffffff) ? j : gi;
     }
     __syncthreads();
     const int p = sh.p;
     const int gp = p / (256 * RPT);
-    if (tid < kLeafW) {
-        const bool c_in = tid < w;
-        sh.prow[tid] = c_in ? a.xrow[(static_cast<size_t>(par) * R + gp) * kLeafW + tid] : T(0);
-        sh.jrow[tid] = c_in ? a.xrow[(static_cast<size_t>(par) * R + G) * kLeafW + tid] : T(0);
-    }
-    if (tid == 0) {
-        sh.ptag = a.xtag[static_cast<size_t>(par) * R + gp];
-        sh.jtag = a.xtag[static_cast<size_t>(par) * R + G];
-    }
-    __syncthreads();
-
-    const T piv = sh.prow[j];
+    const unsigned int* prec = sh.rec[gp];
+    const unsigned int* jrec = sh.rec[G];
+    const T piv = from_words<T>(prec + R::ROW + R::PW * j);
     if (piv != T(0)) {
         if (p != j) {
 #pragma unroll
@@ -218,17 +308,20 @@ __device__ __attribute__((always_inline)) inline bool leaf_column(
                 const int lrow = base + r * 256 + tid;
                 if (lrow == p) {
 #pragma unroll
-                    for (int c = 0; c < kLeafW; ++c) x[r][c] = sh.jrow[c];
-                    tag[r] = sh.jtag;
+                    for (int c = 0; c < kLeafW; ++c) x[r][c] = from_words<T>(jrec + R::ROW + R::PW * c);
+                    tag[r] = static_cast<int>(jrec[R::TAG]);
                 } else if (lrow == j) {
 #pragma unroll
-                    for (int c = 0; c < kLeafW; ++c) x[r][c] = sh.prow[c];
-                    tag[r] = sh.ptag;
+                    for (int c = 0; c < kLeafW; ++c) x[r][c] = from_words<T>(prec + R::ROW + R::PW * c);
+                    tag[r] = static_cast<int>(prec[R::TAG]);
                 }
             }
         }
         const bool recip = fabs(piv) >= sfmin<T>();
         const T rcp = T(1) / piv;
+        T u[kLeafW];
+#pragma unroll
+        for (int c = j + 1; c < kLeafW; ++c) u[c] = from_words<T>(prec + R::ROW + R::PW * c);
 #pragma unroll
         for (int r = 0; r < RPT; ++r) {
             const int lrow = base + r * 256 + tid;
@@ -236,23 +329,25 @@ __device__ __attribute__((always_inline)) inline bool leaf_column(
                 const T l = recip ? x[r][j] * rcp : x[r][j] / piv;
                 x[r][j] = l;
 #pragma unroll
-                for (int c = j + 1; c < kLeafW; ++c) x[r][c] -= l * sh.prow[c];
+                for (int c = j + 1; c < kLeafW; ++c) x[r][c] -= l * u[c];
             }
         }
     } else if (g == 0 && tid == 0) {
         atomicCAS(a.info, 0, a.c0 + j + 1);
     }
     if (g == 0 && tid == 0) a.ipiv[a.c0 + j] = a.c0 + p + 1;
+    (void)w;
     return true;
 }
 
 template <typename T, int RPT, int... Js>
-__device__ __attribute__((always_inline)) inline void leaf_columns(
+__device__ __attribute__((always_inline)) inline bool leaf_columns(
     const LeafArgs<T>& a, LeafShared<T>& sh, T (&x)[RPT][kLeafW], int (&tag)[RPT], int base,
     std::integer_sequence<int, Js...>)
 {
     bool ok = true;
     ((ok = ok && (Js >= a.w || leaf_column<T, RPT, Js>(a, sh, x, tag, base))), ...);
+    return ok;
 }
 
 template <typename T, int RPT>
@@ -261,8 +356,8 @@ __global__ __launch_bounds__(256) void getf2_leaf_kernel(LeafArgs<T> a)
     const int tid = threadIdx.x;
     const int base = blockIdx.x * 256 * RPT;
     __shared__ LeafShared<T> sh;
-    // A leaf whose predecessor timed out leaves the grid-barrier counter out
-    // of step: every later leaf exits at once (the host reports the timeout).
+    // A leaf whose predecessor timed out would wait on stale records: every
+    // later leaf exits at once (the host reports the timeout).
     if (__hip_atomic_load(a.status, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0) return;
 
     T x[RPT][kLeafW];
@@ -276,11 +371,9 @@ __global__ __launch_bounds__(256) void getf2_leaf_kernel(LeafArgs<T> a)
             x[r][c] = (in && c < a.w) ? a.A[off(a.c0 + lrow, a.c0 + c, a.lda)] : T(0);
         tag[r] = in ? a.orig[a.c0 + lrow] : -1;
     }
-    if (tid == 0) sh.abort = 0;
-    __syncthreads();
 
-    leaf_columns<T, RPT>(a, sh, x, tag, base, std::make_integer_sequence<int, kLeafW>{});
-    if (sh.abort) return;
+    if (!leaf_columns<T, RPT>(a, sh, x, tag, base, std::make_integer_sequence<int, kLeafW>{}))
+        return;
 
 #pragma unroll
     for (int r = 0; r < RPT; ++r) {
@@ -308,7 +401,7 @@ static void launch_leaf(mifact_handle h, LeafArgs<T>& args)
         report_hip(h, e, "hipLaunchCooperativeKernel(getf2_leaf)", __FILE__, __LINE__);
         return;
     }
-    h->barrier_base += static_cast<unsigned long long>(args.w) * args.G;
+    h->leaf_epoch += static_cast<unsigned int>(args.w);
 }
 
 template <typename T>
@@ -332,22 +425,16 @@ void getf2_leaf(mifact_handle h, T* A, long long lda, int n, int c0, int w,
     args.ipiv = ipiv;
     args.orig = orig;
     args.info = info;
-    char* ex = reinterpret_cast<char*>(ws.exchange);
-    const size_t R = kMaxG + 1;
-    args.xval = reinterpret_cast<T*>(ex);
-    args.xrow = args.xval + 2 * R;
-    args.xidx = reinterpret_cast<int*>(args.xrow + 2 * R * kLeafW);
-    args.xtag = args.xidx + 2 * R;
-    args.bar = h->barrier;
-    args.bar_base = h->barrier_base;
+    args.xrec = reinterpret_cast<unsigned long long*>(ws.exchange);
+    args.epoch0 = h->leaf_epoch;
     args.status = h->kernel_status;
-    // Rows per thread: keep G (the number of workgroups that exchange per
-    // column) at 64 or below where possible.
+    // Rows per thread: keep G (the workgroups that exchange per column) at 64
+    // or below where the registers allow.
     int rpt = 1;
     while (rpt < 4 && ceil_div(m, 256 * rpt) > 64) rpt *= 2;
     args.G = ceil_div(m, 256 * rpt);
     if (args.G > kMaxG) {
-        report_status(h, mifact_status_invalid_value, "getf2_leaf: panel too tall");
+        report_status(h, mifact_status_invalid_value, "getf2_leaf: panel taller than 131072 rows");
         return;
     }
     if (rpt == 1) launch_leaf<T, 1>(h, args);

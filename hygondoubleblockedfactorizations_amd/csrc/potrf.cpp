@@ -35,7 +35,8 @@ void double_blocked_trsm_impl(mifact_handle h, const T* L, long long ldl, T* B,
         for (int inner = outer; inner < outer_end; inner += b) {
             const int inner_size = imin(b, outer_end - inner);
             const int panel_start = inner + inner_size;
-            trsm_rlt<T>(h, n, inner_size, at(L, ldl, inner, inner), ldl, at(B, ldb, 0, inner), ldb);
+            trsm_rlt<T>(h, n, inner_size, at(L, ldl, inner, inner), ldl, at(B, ldb, 0, inner),
+                        ldb);
             if (panel_start < outer_end) {
                 const int pc = imin(b, outer_end - panel_start);
                 gemm<T>(h, kOpNT, kShapeFull, n, pc, panel_start - outer, T(-1),
@@ -64,6 +65,8 @@ void double_blocked_potrf_impl(mifact_handle h, T* A, int lda_i, int n, int nb, 
     if (b <= 0) b = nb;
     b = imin(b, nb);
     const long long lda = lda_i;
+    if (!ensure_workspace(h, n)) return;
+    T* inv = invdiag<T>(h);
     if (info != nullptr) clear_int(h, info);
 
     for (int outer = 0; outer < n; outer += nb) {
@@ -74,8 +77,9 @@ void double_blocked_potrf_impl(mifact_handle h, T* A, int lda_i, int n, int nb, 
             const int panel_start = inner + inner_size;
             const int trailing_rows = n - panel_start;
             T* Aii = at(A, lda, inner, inner);
-            // diagonal block (rocsolver_potrf at :334)
-            potrf_diag<T>(h, inner_size, Aii, lda, info, inner);
+            // diagonal block (rocsolver_potrf at :334); its 64-wide leaves
+            // also leave the inverses of their blocks in `inv`
+            potrf_diag<T>(h, inner_size, Aii, lda, info, inner, inv);
             // panel solve (:336-354)
             if (trailing_rows > 0) {
                 if (use_dbtrsm) {
@@ -83,8 +87,8 @@ void double_blocked_potrf_impl(mifact_handle h, T* A, int lda_i, int n, int nb, 
                                                 trailing_rows, inner_size, inner_size,
                                                 imin(kDbTrsmMicroBlock, inner_size));
                 } else {
-                    trsm_rlt<T>(h, trailing_rows, inner_size, Aii, lda,
-                                at(A, lda, panel_start, inner), lda);
+                    trsm_rlt_inv<T>(h, trailing_rows, inner_size, Aii, lda,
+                                    at(A, lda, panel_start, inner), lda, inv);
                 }
             }
             // next inner panel -= L * L^T over the outer block's columns (:210-249);
@@ -141,8 +145,10 @@ void gemm_nn_update_impl(mifact_handle h, int rows, int cols, int k, const T* A,
 template <typename T>
 void potrf_block_impl(mifact_handle h, int n, T* A, int lda, int* info)
 {
+    if (n <= 0) return;
+    if (!ensure_workspace(h, n)) return;
     if (info != nullptr) clear_int(h, info);
-    potrf_diag<T>(h, n, A, lda, info, 0);
+    potrf_diag<T>(h, n, A, lda, info, 0, invdiag<T>(h));
 }
 
 template <typename T>

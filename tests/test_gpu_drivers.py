@@ -92,15 +92,18 @@ def run_getrf(H, A, onb, inb, mode=1, tb=0, batched=False, dtype=np.float32):
 
 
 def exact_spd(n, seed):
+    """A = L L^T, small-integer L whose entries within 64 of the diagonal are
+    zero (every 64-wide diagonal block is diag(1|2), so its inverse -- what
+    the triangular-solve leaves multiply by -- is exact too)."""
     r = np.random.default_rng(seed)
-    L = np.tril(r.integers(-2, 3, size=(n, n)), -1).astype(np.float64)
+    L = np.tril(r.integers(-2, 3, size=(n, n)), -64).astype(np.float64)
     L += np.diag(r.choice([1.0, 2.0], size=n))
     return np.asfortranarray(L.astype(np.float32)), np.asfortranarray((L @ L.T).astype(np.float32))
 
 
 def exact_lu(n, seed):
     r = np.random.default_rng(seed)
-    L = np.tril(r.choice([-0.5, 0.0, 0.5], size=(n, n)), -1) + np.eye(n)
+    L = np.tril(r.choice([-0.5, 0.0, 0.5], size=(n, n)), -64) + np.eye(n)
     U = np.triu(r.integers(-3, 4, size=(n, n)), 1).astype(np.float64)
     U += np.diag(r.choice([1.0, -1.0, 2.0, -2.0, 4.0], size=n))
     perm = r.permutation(n)

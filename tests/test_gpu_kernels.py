@@ -135,8 +135,11 @@ def test_double_blocked_trsm(H):
 
 
 def exact_spd(n, seed):
+    """A = L L^T, small-integer L whose entries within 64 of the diagonal are
+    zero (every 64-wide diagonal block is diag(1|2), so its inverse -- what
+    the triangular-solve leaves multiply by -- is exact too)."""
     r = np.random.default_rng(seed)
-    L = np.tril(r.integers(-2, 3, size=(n, n)), -1).astype(np.float64)
+    L = np.tril(r.integers(-2, 3, size=(n, n)), -64).astype(np.float64)
     L += np.diag(r.choice([1.0, 2.0], size=n))
     return np.asfortranarray(L.astype(np.float32)), np.asfortranarray((L @ L.T).astype(np.float32))
 
@@ -150,6 +153,22 @@ def test_potrf_block_closed_form_exact(H, n):
     sync()
     assert int(info[0]) == 0
     assert np.array_equal(np.tril(host(dA)), L)
+
+
+@pytest.mark.parametrize("n", [5, 33, 64])
+def test_potf2_leaf_elimination_exact(H, n):
+    """One 64-wide leaf (no inverse applied): integer L with entries next to
+    the diagonal must come back bit for bit."""
+    log("potf2 leaf exact", n)
+    r = np.random.default_rng(n)
+    L = np.tril(r.integers(-2, 3, size=(n, n)), -1).astype(np.float64)
+    L += np.diag(r.choice([1.0, 2.0], size=n))
+    A = np.asfortranarray((L @ L.T).astype(np.float32))
+    dA, info = dev(A), dev_int(1, 7)
+    lib().mifact_potrf_block(H.ptr, n, P(dA), n, P(info))
+    sync()
+    assert int(info[0]) == 0
+    assert np.array_equal(np.tril(host(dA)), L.astype(np.float32))
 
 
 @pytest.mark.parametrize("n,bad", [(64, 10), (300, 150), (1000, 999)])
@@ -204,10 +223,12 @@ def test_iamax_large_with_ties(H, n):
     assert int(res[0]) == n // 3 + 1
 
 
-def exact_lu(m, n, seed):
-    """P A = L U, |l_ij| <= 1/2, power-of-two diag(U): exact in FP32."""
+def exact_lu(m, n, seed, band=64):
+    """P A = L U, |l_ij| <= 1/2, power-of-two diag(U): exact in FP32.  L is
+    zero within `band` of the diagonal (exact 64-block inverses); band=1 is
+    used for single-leaf panels, where no inverse is applied."""
     r = np.random.default_rng(seed)
-    L = np.tril(r.choice([-0.5, 0.0, 0.5], size=(m, n)), -1)
+    L = np.tril(r.choice([-0.5, 0.0, 0.5], size=(m, n)), -band)
     L[:n, :n] += np.eye(n)
     U = np.triu(r.integers(-3, 4, size=(n, n)), 1).astype(np.float64)
     U += np.diag(r.choice([1.0, -1.0, 2.0, -2.0, 4.0], size=n))
@@ -217,10 +238,11 @@ def exact_lu(m, n, seed):
     return np.asfortranarray(A.astype(np.float32))
 
 
-@pytest.mark.parametrize("m,n", [(32, 32), (100, 20), (700, 64), (5000, 96), (40000, 40)])
-def test_getrf_panel_closed_form(H, m, n):
-    log("getrf_panel exact", m, n)
-    A = exact_lu(m, n, m + n)
+@pytest.mark.parametrize("m,n,band", [(32, 32, 1), (100, 20, 1), (3000, 32, 1), (700, 64, 64),
+                                      (5000, 96, 64), (40000, 40, 64)])
+def test_getrf_panel_closed_form(H, m, n, band):
+    log("getrf_panel exact", m, n, band)
+    A = exact_lu(m, n, m + n, band)
     X = A.copy(order="F")
     ipiv_ref, info_ref = O.getf2(X)
     dA, ipiv, info = dev(A), dev_int(n), dev_int(1, 5)
