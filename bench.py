@@ -54,6 +54,13 @@ def parse():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-verify", action="store_true")
     ap.add_argument("--no-profile", action="store_true")
+    ap.add_argument("--no-solve", action="store_true",
+                    help="skip the mixed+IR vs FP32 solve comparison")
+    ap.add_argument("--solve-precision", default="fp16", choices=["bf16", "fp16"])
+    ap.add_argument("--solve-kappa", type=float, default=1e4,
+                    help="condition number of the SPD solve matrix Q diag(sigma) Q^T")
+    ap.add_argument("--no-batched", action="store_true", help="skip the batched section")
+    ap.add_argument("--batch-spd", type=int, default=4096)
     return ap.parse_args()
 
 
@@ -161,6 +168,185 @@ def cpu_baseline(torch, args):
         "potrf_tflops": n ** 3 / 3.0 / (t1 - t0) / 1e12,
         "getrf_tflops": 2.0 * n ** 3 / 3.0 / (t2 - t1) / 1e12,
     }
+
+
+def solve_compare(torch, lib, h, P, n, routine, A, nb, b, tb, prec, seed):
+    """Time-to-solution of the plain FP32 solve (FP32 factorization + FP32
+    triangular solves) against the mixed-precision factorization with FP64
+    refinement run to the FP32 solve's own backward error."""
+    import ctypes
+    from hygondoubleblockedfactorizations_amd import _lib
+    g = torch.Generator(device="cuda")
+    g.manual_seed(seed)
+    xt = torch.randn(n, 1, dtype=torch.float64, device="cuda", generator=g)
+    # b = A x_true in FP64 (A stored FP32; column-major buffer = row-major A^T)
+    b64 = (A.t().double() @ xt) if routine == "getrf" else (A.double() @ xt)
+    info = torch.zeros(1, dtype=torch.int32, device="cuda")
+    ipiv = torch.zeros(n, dtype=torch.int32, device="cuda")
+    work = torch.empty_like(A)
+    ev0, ev1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    stream = torch.cuda.current_stream()
+
+    def berr_of(x64):
+        r = torch.empty_like(b64)
+        lib.mifact_residual_fp64(h.ptr, 1 if routine == "potrf" else 0, n, 1, P(A), n, P(x64), n,
+                                 P(b64), n, P(r), n)
+        torch.cuda.synchronize()
+        if routine == "potrf":
+            anorm = (A.abs().sum(0)).max().item()      # symmetric
+        else:
+            anorm = (A.abs().sum(0)).max().item()      # buffer column sums = rows of A
+        return r.abs().max().item() / (anorm * x64.abs().max().item() + b64.abs().max().item())
+
+    # plain FP32 solve
+    out = {}
+    for rep in range(2):
+        work.copy_(A)
+        x32 = b64.float().clone()
+        torch.cuda.synchronize()
+        ev0.record(stream)
+        if routine == "potrf":
+            lib.double_blocked_potrf(h.ptr, P(work), n, n, nb, b, P(info))
+            lib.mifact_potrs(h.ptr, n, 1, P(work), n, P(x32), n)
+        else:
+            lib.double_blocked_getrf(h.ptr, P(work), n, n, nb, b, 1, tb, P(ipiv), P(info))
+            lib.mifact_getrs(h.ptr, n, 1, P(work), n, P(ipiv), P(x32), n)
+        ev1.record(stream)
+        torch.cuda.synchronize()
+        t32 = ev0.elapsed_time(ev1)
+    berr32 = berr_of(x32.double())
+    fwd32 = ((x32.double() - xt).abs().max() / xt.abs().max()).item()
+    out["fp32_solve"] = {"ms": t32, "backward_error": berr32, "forward_error": fwd32,
+                         "info": int(info[0])}
+    log(f"{routine} FP32 solve {t32:.1f} ms berr {berr32:.2e}")
+    # mixed + FP64 refinement to the same backward error
+    x = torch.zeros_like(b64)
+    prm = _lib.RefineParams(_lib.PRECISION[prec], nb, b, 1, tb, 30, berr32, 1,
+                            _lib.REFINE_METHOD["auto"])
+    res = _lib.RefineResult()
+    for rep in range(2):
+        torch.cuda.synchronize()
+        ev0.record(stream)
+        if routine == "potrf":
+            st = lib.mifact_posv_refine(h.ptr, n, 1, P(A), n, P(work), n, P(b64), n, P(x), n,
+                                        ctypes.byref(prm), ctypes.byref(res))
+        else:
+            st = lib.mifact_gesv_refine(h.ptr, n, 1, P(A), n, P(work), n, P(ipiv), P(b64), n,
+                                        P(x), n, ctypes.byref(prm), ctypes.byref(res))
+        ev1.record(stream)
+        torch.cuda.synchronize()
+        tm = ev0.elapsed_time(ev1)
+        if st != 0:
+            raise RuntimeError(f"refine status {st}")
+    berr_m = berr_of(x)
+    fwd_m = ((x - xt).abs().max() / xt.abs().max()).item()
+    out["mixed_ir"] = dict(res.as_dict(), ms=tm, precision=prec, backward_error_check=berr_m,
+                           forward_error=fwd_m, tolerance=berr32)
+    out["speedup_time_to_solution"] = t32 / tm
+    log(f"{routine} mixed {prec}+IR {tm:.1f} ms berr {berr_m:.2e} "
+        f"({res.iterations} steps, {res.gmres_iterations} fgmres, fallback {res.fell_back})")
+    return out
+
+
+def spd_kappa(torch, n, kappa, seed):
+    """A = Q diag(sigma) Q^T, Q from the Householder QR of a seeded Gaussian
+    (FP64 on the GPU), sigma_i = kappa^(-i/(n-1)); stored FP32."""
+    g = torch.Generator(device="cuda")
+    g.manual_seed(seed)
+    q, _ = torch.linalg.qr(torch.randn(n, n, dtype=torch.float64, device="cuda", generator=g))
+    sig = kappa ** (-torch.arange(n, dtype=torch.float64, device="cuda") / max(n - 1, 1))
+    a = (q * sig) @ q.t()
+    del q
+    a = 0.5 * (a + a.t())
+    return a.float().contiguous()
+
+
+def batched_section(torch, lib, h, P, batch, seed):
+    """Config: `batch` SPD matrices with n ~ U[32, 512], A = G G^T / n + I,
+    packed column-major.  One persistent launch vs one double_blocked_potrf
+    call per matrix; then the BF16-MFMA factorization + FP64 refinement
+    (graph-captured) with one right-hand side each."""
+    import numpy as np
+    rng = np.random.Generator(np.random.PCG64(seed))
+    ns = rng.integers(32, 513, size=batch).astype(np.int32)
+    sizes = ns.astype(np.int64) ** 2
+    offs = np.concatenate([[0], np.cumsum(sizes)[:-1]]).astype(np.int64)
+    total = int(sizes.sum())
+    g = torch.Generator(device="cuda")
+    g.manual_seed(seed)
+    flat = torch.empty(total, dtype=torch.float32, device="cuda")
+    for i in range(batch):
+        n = int(ns[i])
+        G = torch.randn(n, n, device="cuda", generator=g, dtype=torch.float64)
+        a = (G @ G.t() / n + torch.eye(n, device="cuda", dtype=torch.float64)).float()
+        flat[offs[i]:offs[i] + n * n] = a.reshape(-1)
+    d_off = torch.from_numpy(offs).cuda()
+    d_n = torch.from_numpy(ns).cuda()
+    info = torch.zeros(batch, dtype=torch.int32, device="cuda")
+    work = torch.empty_like(flat)
+    ev0, ev1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    stream = torch.cuda.current_stream()
+    flops = float((ns.astype(np.float64) ** 3).sum() / 3.0)
+    out = {"workload": f"{batch} SPD matrices, n ~ U[32,512], FP32, packed column-major",
+           "flops": flops}
+    t_b = []
+    for rep in range(3):
+        work.copy_(flat)
+        torch.cuda.synchronize()
+        ev0.record(stream)
+        lib.mifact_potrf_batched(h.ptr, batch, P(work), P(d_off), P(d_n), P(d_n), P(info))
+        ev1.record(stream)
+        torch.cuda.synchronize()
+        t_b.append(ev0.elapsed_time(ev1))
+    out["batched_ms"] = float(min(t_b))
+    out["batched_tflops"] = flops / (out["batched_ms"] * 1e-3) / 1e12
+    out["batched_info_nonzero"] = int((info != 0).sum())
+    work.copy_(flat)
+    info1 = torch.zeros(1, dtype=torch.int32, device="cuda")
+    torch.cuda.synchronize()
+    ev0.record(stream)
+    base = work.data_ptr()
+    import ctypes
+    for i in range(batch):
+        n = int(ns[i])
+        lib.double_blocked_potrf(h.ptr, ctypes.c_void_p(base + 4 * int(offs[i])), n, n, 256, 64,
+                                 P(info1))
+    ev1.record(stream)
+    torch.cuda.synchronize()
+    out["per_matrix_ms"] = ev0.elapsed_time(ev1)
+    out["speedup_vs_per_matrix"] = out["per_matrix_ms"] / out["batched_ms"]
+    log(f"batched POTRF {out['batched_ms']:.2f} ms vs per-matrix {out['per_matrix_ms']:.1f} ms")
+    # BF16-MFMA + FP64 refinement (graph-captured), one rhs per matrix
+    voff = np.concatenate([[0], np.cumsum(ns)[:-1]]).astype(np.int64)
+    nvec = int(ns.sum())
+    xt = torch.randn(nvec, dtype=torch.float64, device="cuda", generator=g)
+    bvec = torch.empty_like(xt)
+    for i in range(batch):
+        n = int(ns[i])
+        a = flat[offs[i]:offs[i] + n * n].view(n, n).double()
+        bvec[voff[i]:voff[i] + n] = a.t() @ xt[voff[i]:voff[i] + n]
+    x = torch.zeros_like(xt)
+    berr = torch.zeros(batch, dtype=torch.float64, device="cuda")
+    d_voff = torch.from_numpy(voff).cuda()
+    t_r = []
+    for rep in range(2):
+        torch.cuda.synchronize()
+        ev0.record(stream)
+        st = lib.mifact_gesv_batched_refine(h.ptr, 1, 1, batch, P(flat), P(work), P(d_off),
+                                            P(d_n), P(d_n), None, None, P(bvec), P(x), P(d_voff),
+                                            nvec, 10, 1e-12, P(berr), P(info))
+        ev1.record(stream)
+        torch.cuda.synchronize()
+        t_r.append(ev0.elapsed_time(ev1))
+        if st != 0:
+            raise RuntimeError(f"batched refine status {st}")
+    be = berr.cpu().numpy()
+    out["bf16_ir"] = {"ms": float(min(t_r)), "iterations_captured": 10, "tolerance": 1e-12,
+                      "max_backward_error": float(be.max()),
+                      "converged_fraction": float((be <= 1e-12).mean())}
+    out["lu"] = "batched LU switched off (see DESIGN.md)"
+    log(f"batched BF16+IR {out['bf16_ir']['ms']:.2f} ms max berr {be.max():.2e}")
+    return out
 
 
 def main():
@@ -280,6 +466,34 @@ def main():
                      "gbs": (v["bytes"] / (v["ms"] * 1e-3) / 1e9) if v["ms"] > 0 else None}
                  for k, v in stages.items()}
 
+    solve = None
+    if not args.no_solve:
+        solve = {}
+        try:
+            for r in routines:
+                if r == "potrf":
+                    t_gen = time.perf_counter()
+                    A = spd_kappa(torch, n, args.solve_kappa, args.seed + 7)
+                    log(f"kappa={args.solve_kappa:g} SPD matrix in {time.perf_counter() - t_gen:.1f} s")
+                    solve[r] = solve_compare(torch, lib, h, P, n, r, A, pnb, pb, tb,
+                                             args.solve_precision, args.seed + 8)
+                    solve[r]["matrix"] = f"Q diag(sigma) Q^T, kappa={args.solve_kappa:g}"
+                    del A
+                else:
+                    solve[r] = solve_compare(torch, lib, h, P, n, r, inputs[r], gnb, gb, tb,
+                                             args.solve_precision, args.seed + 9)
+                    solve[r]["matrix"] = "iid Uniform(-1,1)"
+        except Exception as e:  # report, never hide the GPU line
+            log(f"solve comparison failed: {e}")
+            solve["error"] = str(e)
+    batched = None
+    if not args.no_batched:
+        try:
+            batched = batched_section(torch, lib, h, P, args.batch_spd, args.seed + 11)
+        except Exception as e:
+            log(f"batched section failed: {e}")
+            batched = {"error": str(e)}
+
     cpu = None
     if not args.no_cpu_baseline:
         try:
@@ -308,6 +522,8 @@ def main():
         "roofline": roofline,
         "cpu_baseline": cpu,
         "stages_per_step": stage_out,
+        "solve": solve,
+        "batched": batched,
     }
     print(json.dumps(line), flush=True)
 

@@ -25,6 +25,7 @@ SIGNATURES = {
     "mifact_set_error_mode": [P, I],
     "mifact_get_last_error": [P],
     "mifact_reserve_workspace": [P, I],
+    "mifact_set_gemm_config": [P, I],
     "mifact_version": [],
     "mifact_profile_begin": [P],
     "mifact_profile_end": [P, P, I, ctypes.POINTER(I)],
@@ -74,6 +75,22 @@ SIGNATURES = {
     "mifact_getrf_panel_double": [P, I, I, P, I, P, P],
     "mifact_isamax": [P, I, P, I, P],
     "mifact_idamax": [P, I, P, I, P],
+    # solve/mixed_refine.h
+    "mifact_gemm_update_lowp": [P, I, I, I, I, I, I, ctypes.c_float, P, I, P, I, P, I],
+    "double_blocked_potrf_mixed": [P, I, P, I, I, I, I, P],
+    "double_blocked_getrf_mixed": [P, I, P, I, I, I, I, I, I, P, P],
+    "mifact_potrs": [P, I, I, P, I, P, I],
+    "mifact_getrs": [P, I, I, P, I, P, P, I],
+    "mifact_residual_fp64": [P, I, I, I, P, I, P, I, P, I, P, I],
+    "mifact_posv_refine": [P, I, I, P, I, P, I, P, I, P, I, P, P],
+    "mifact_gesv_refine": [P, I, I, P, I, P, I, P, P, I, P, I, P, P],
+    # batched/batched_factorizations.h
+    "mifact_potrf_batched": [P, I, P, P, P, P, P],
+    "mifact_getrf_batched": [P, I, P, P, P, P, P, P, P],
+    "mifact_potrf_batched_ex": [P, I, I, P, P, P, P, P],
+    "mifact_getrf_batched_ex": [P, I, I, P, P, P, P, P, P, P],
+    "mifact_gesv_batched_refine": [P, I, I, I, P, P, P, P, P, P, P, P, P, P,
+                                   ctypes.c_longlong, I, ctypes.c_double, P, P],
 }
 
 _RESTYPE = {
@@ -84,10 +101,19 @@ _RESTYPE = {
     "mifact_set_error_mode": I,
     "mifact_get_last_error": I,
     "mifact_reserve_workspace": I,
+    "mifact_set_gemm_config": I,
     "mifact_version": ctypes.c_char_p,
     "mifact_profile_begin": I,
     "mifact_profile_end": I,
+    "mifact_posv_refine": I,
+    "mifact_gesv_refine": I,
+    "mifact_gesv_batched_refine": I,
 }
+
+# mifact_precision (include/solve/mixed_refine.h)
+PRECISION = {"fp32": 0, "bf16": 1, "fp16": 2}
+# mifact_refine_method
+REFINE_METHOD = {"classical": 0, "gmres": 1, "auto": 2}
 
 _lib = None
 
@@ -211,3 +237,38 @@ def ptr(t):
     if hasattr(t, "data_ptr"):
         return P(t.data_ptr())
     return P(t.ctypes.data)
+
+
+class RefineParams(ctypes.Structure):
+    """Layout of mifact_refine_params (include/solve/mixed_refine.h)."""
+
+    _fields_ = [
+        ("precision", I),
+        ("nb", I),
+        ("b", I),
+        ("outer_trsm_mode", I),
+        ("trsm_block", I),
+        ("max_iterations", I),
+        ("tolerance", ctypes.c_double),
+        ("fallback_fp32", I),
+        ("method", I),
+    ]
+
+
+class RefineResult(ctypes.Structure):
+    """Layout of mifact_refine_result (include/solve/mixed_refine.h)."""
+
+    _fields_ = [
+        ("info", I),
+        ("iterations", I),
+        ("converged", I),
+        ("backward_error", ctypes.c_double),
+        ("first_backward_error", ctypes.c_double),
+        ("fell_back", I),
+        ("factorizations", I),
+        ("gmres_iterations", I),
+        ("diagonal_shift", ctypes.c_double),
+    ]
+
+    def as_dict(self):
+        return {f: getattr(self, f) for f, _ in self._fields_}

@@ -33,10 +33,12 @@ SOURCES = [
     "pivot.hip",
     "potrf.cpp",
     "getrf.cpp",
+    "refine.hip",
+    "batched.hip",
 ]
 
 COMMON_FLAGS = [
-    "-O3", "-std=c++17", "-fPIC", f"--offload-arch={ARCH}",
+    "-O3", "-std=c++20", "-fPIC", f"--offload-arch={ARCH}",
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
     "-Wall", "-Wno-unused-function", "-Wno-unused-variable",
     f"-I{INCLUDE}", f"-I{CSRC}",

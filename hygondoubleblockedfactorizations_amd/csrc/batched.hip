@@ -1,0 +1,786 @@
+// Uneven batches of small Cholesky / LU factorizations in one persistent
+// launch, and batched FP64 refinement solves captured in a hipGraph.
+//
+// Each workgroup (4 waves) loops: take the next matrix id from a device
+// work queue (matrices ordered largest first), factor it right-looking with
+// 32-column panels, repeat until the queue is drained.  Per panel:
+//   POTRF: POTF2 of the 32 x 32 diagonal block in LDS, row-parallel TRSM of
+//          the rows below, lower-trapezoid SYRK of the trailing matrix;
+//   GETRF: column-by-column partial pivoting on the panel (smallest index on
+//          ties, LAPACK sgetf2 semantics), full-row exchanges,
+//          column-parallel unit-lower TRSM of the block row, GEMM update.
+// The trailing updates run as 64 x 64 MFMA tiles (v_mfma_f32_32x32x2_f32,
+// or v_mfma_f32_32x32x16_bf16/_f16 with FP32 accumulation for the
+// mixed-precision variants).  Same arithmetic contract as the large drivers
+// (reference: src/potrf/double_blocked_potrf.cpp:294-381,
+// src/getrf/double_blocked_getrf.cpp:125-191), one matrix per workgroup.
+#include <hip/hip_runtime.h>
+
+#include <cfloat>
+#include <cmath>
+#include <cstring>
+
+#include "batched/batched_factorizations.h"
+#include "kernels.hpp"
+
+namespace mifact {
+namespace {
+
+constexpr int kNB = 32;             // panel width
+constexpr int kT = 64;              // trailing-update tile
+constexpr int kMaxOrder = 4096;
+constexpr int kBuckets = 256;       // work-queue size buckets (16 orders each)
+
+typedef float floatx16 __attribute__((ext_vector_type(16)));
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef _Float16 halfx8 __attribute__((ext_vector_type(8)));
+
+struct BatchArgs {
+    float* A;
+    const long long* offsets;
+    const int* n;
+    const int* lda;
+    int* ipiv;
+    const long long* ipiv_offsets;
+    int* info;
+    const int* order;
+    unsigned* counter;
+    int batch;
+};
+
+struct TileShared {
+    float a[kNB][kT + 1];    // a[k][row]: left operand (rows of the panel)
+    float b[kNB][kT + 1];    // b[k][col]: right operand
+};
+
+// C(r0:r0+64, c0:c0+64) -= L(r0:, 0:kb) R(0:kb, c0:)   on MFMA.
+// la(i, k): left operand element (row r0 + i); rb(k, j): right operand
+// element (column c0 + j).  LOWER: only C(i, j) with global row >= column.
+template <int PREC, bool LOWER, typename LA, typename RB>
+__device__ void tile_update(float* C, long long ldc, int m, int ncols, int r0, int c0, int kb,
+                            LA la, RB rb, TileShared& s)
+{
+    const int tid = threadIdx.x;
+    const int lane = tid & 63, wave = tid >> 6;
+    const int wm = wave & 1, wn = wave >> 1;
+    // stage (zero beyond the matrix and beyond kb)
+    for (int e = tid; e < kNB * kT; e += 256) {
+        const int k = e / kT, i = e % kT;
+        float va = 0.0f, vb = 0.0f;
+        if (k < kb) {
+            if (r0 + i < m) va = la(r0 + i, k);
+            if (c0 + i < ncols) vb = rb(k, c0 + i);
+        }
+        s.a[k][i] = va;
+        s.b[k][i] = vb;
+    }
+    __syncthreads();
+    floatx16 acc;
+#pragma unroll
+    for (int r = 0; r < 16; ++r) acc[r] = 0.0f;
+    const int row = wm * 32 + (lane & 31);
+    const int col = wn * 32 + (lane & 31);
+    if (PREC == kPrecFP32) {
+#pragma unroll
+        for (int kk = 0; kk < kNB; kk += 2) {
+            const float a = s.a[kk + (lane >> 5)][row];
+            const float b = s.b[kk + (lane >> 5)][col];
+            acc = __builtin_amdgcn_mfma_f32_32x32x2f32(a, b, acc, 0, 0, 0);
+        }
+    } else {
+#pragma unroll
+        for (int kk = 0; kk < kNB; kk += 16) {
+            const int k0 = kk + 8 * (lane >> 5);
+            if (PREC == kPrecBF16) {
+                bf16x8 a, b;
+#pragma unroll
+                for (int e = 0; e < 8; ++e) {
+                    a[e] = static_cast<__bf16>(s.a[k0 + e][row]);
+                    b[e] = static_cast<__bf16>(s.b[k0 + e][col]);
+                }
+                acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, b, acc, 0, 0, 0);
+            } else {
+                halfx8 a, b;
+#pragma unroll
+                for (int e = 0; e < 8; ++e) {
+                    a[e] = static_cast<_Float16>(s.a[k0 + e][row]);
+                    b[e] = static_cast<_Float16>(s.b[k0 + e][col]);
+                }
+                acc = __builtin_amdgcn_mfma_f32_32x32x16_f16(a, b, acc, 0, 0, 0);
+            }
+        }
+    }
+    // accumulator r: row 8 (r / 4) + 4 (lane / 32) + r % 4, column lane % 32
+    const int gc = c0 + col;
+    if (gc < ncols) {
+#pragma unroll
+        for (int r = 0; r < 16; ++r) {
+            const int gr = r0 + wm * 32 + 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3);
+            if (gr < m && (!LOWER || gr >= gc)) C[off(gr, gc, ldc)] -= acc[r];
+        }
+    }
+    __syncthreads();
+}
+
+// ---------------------------------------------------------------- Cholesky
+
+template <int PREC>
+__device__ void chol_matrix(float* A, long long lda, int n, int* info)
+{
+    __shared__ float d[kNB][kNB + 1];
+    __shared__ TileShared ts;
+    __shared__ int failed;
+    const int tid = threadIdx.x;
+    if (tid == 0) failed = 0;
+    for (int k = 0; k < n; k += kNB) {
+        const int kb = imin(kNB, n - k);
+        // diagonal block -> LDS (lower)
+        for (int e = tid; e < kNB * kNB; e += 256) {
+            const int i = e % kNB, j = e / kNB;
+            d[i][j] = (i < kb && j < kb && i >= j) ? A[off(k + i, k + j, lda)] : 0.0f;
+        }
+        __syncthreads();
+        // POTF2 (right-looking within the block)
+        for (int j = 0; j < kb; ++j) {
+            const float djj = d[j][j];
+            const bool bad = !(djj > 0.0f) || !isfinite(djj);
+            if (bad) {
+                if (tid == 0) {
+                    failed = 1;
+                    *info = k + j + 1;
+                }
+                break;
+            }
+            const float s = sqrtf(djj);
+            __syncthreads();
+            if (tid < kb) {
+                if (tid == j) d[j][j] = s;
+                else if (tid > j) d[tid][j] = d[tid][j] / s;
+            }
+            __syncthreads();
+            for (int e = tid; e < kNB * kNB; e += 256) {
+                const int i = e % kNB, c = e / kNB;
+                if (c > j && i >= c && i < kb) d[i][c] -= d[i][j] * d[c][j];
+            }
+            __syncthreads();
+        }
+        __syncthreads();
+        // write back the factored block (the columns done before a failure)
+        for (int e = tid; e < kNB * kNB; e += 256) {
+            const int i = e % kNB, j = e / kNB;
+            if (i < kb && j < kb && i >= j) A[off(k + i, k + j, lda)] = d[i][j];
+        }
+        __syncthreads();
+        if (failed) return;
+        const int r1 = k + kb;
+        const int m = n - r1;
+        if (m <= 0) break;
+        // TRSM: A(r, k:k+kb) := A(r, k:k+kb) L^{-T}, one thread per row
+        for (int r = r1 + tid; r < n; r += 256) {
+            float x[kNB];
+#pragma unroll
+            for (int j = 0; j < kNB; ++j) x[j] = j < kb ? A[off(r, k + j, lda)] : 0.0f;
+#pragma unroll
+            for (int j = 0; j < kNB; ++j) {
+                if (j < kb) {
+                    float v = x[j];
+#pragma unroll
+                    for (int c = 0; c < j; ++c) v -= x[c] * d[j][c];
+                    x[j] = v / d[j][j];
+                }
+            }
+#pragma unroll
+            for (int j = 0; j < kNB; ++j)
+                if (j < kb) A[off(r, k + j, lda)] = x[j];
+        }
+        __syncthreads();
+        // SYRK: A22 (lower) -= P P^T with P = A(r1:n, k:k+kb)
+        float* C = A + off(r1, r1, lda);
+        const float* Pp = A + off(r1, k, lda);
+        auto la = [&](int i, int kk) { return Pp[off(i, kk, lda)]; };
+        auto rb = [&](int kk, int j) { return Pp[off(j, kk, lda)]; };
+        const int tiles = (m + kT - 1) / kT;
+        for (int tj = 0; tj < tiles; ++tj)
+            for (int ti = tj; ti < tiles; ++ti)
+                tile_update<PREC, true>(C, lda, m, m, ti * kT, tj * kT, kb, la, rb, ts);
+    }
+}
+
+// ---------------------------------------------------------------- LU
+
+template <int PREC>
+__device__ void lu_matrix(float* A, long long lda, int n, int* ipiv, int* info)
+{
+    __shared__ float wv[4];
+    __shared__ int wi[4];
+    __shared__ float prow[kNB];
+    __shared__ float l11[kNB][kNB + 1];
+    __shared__ TileShared ts;
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    constexpr float sfmin = FLT_MIN;
+    for (int k = 0; k < n; k += kNB) {
+        const int kb = imin(kNB, n - k);
+        // panel columns k .. k + kb
+        for (int j = k; j < k + kb; ++j) {
+            // arg max |A(i, j)|, i >= j (smallest index on ties; NaN never wins)
+            float bv = -1.0f;
+            int bi = n;
+            for (int i = j + tid; i < n; i += 256) {
+                const float v = fabsf(A[off(i, j, lda)]);
+                if (v > bv) {
+                    bv = v;
+                    bi = i;
+                }
+            }
+#pragma unroll
+            for (int o = 32; o > 0; o >>= 1) {
+                const float ov = __shfl_xor(bv, o, 64);
+                const int oi = __shfl_xor(bi, o, 64);
+                if (ov > bv || (ov == bv && oi < bi)) {
+                    bv = ov;
+                    bi = oi;
+                }
+            }
+            if (lane == 0) {
+                wv[wave] = bv;
+                wi[wave] = bi;
+            }
+            __syncthreads();
+            float pv = wv[0];
+            int p = wi[0];
+            for (int w = 1; w < 4; ++w)
+                if (wv[w] > pv || (wv[w] == pv && wi[w] < p)) {
+                    pv = wv[w];
+                    p = wi[w];
+                }
+            if (p >= n) p = j;   // all NaN: keep the diagonal
+            if (tid == 0) ipiv[j] = p + 1;
+            const float piv = A[off(p, j, lda)];
+            const bool zero = (piv == 0.0f);
+            if (zero && tid == 0 && *info == 0) *info = j + 1;
+            __syncthreads();     // every thread has read the pivot before rows move
+            // exchange rows j and p over all n columns
+            if (p != j) {
+                for (int c = tid; c < n; c += 256) {
+                    const float t = A[off(j, c, lda)];
+                    A[off(j, c, lda)] = A[off(p, c, lda)];
+                    A[off(p, c, lda)] = t;
+                }
+            }
+            __syncthreads();
+            // pivot row within the panel (columns j+1 .. k+kb)
+            if (tid < kNB) prow[tid] = (j + 1 + tid < k + kb) ? A[off(j, j + 1 + tid, lda)] : 0.0f;
+            __syncthreads();
+            // scale the column and update the rest of the panel
+            if (!zero) {
+                const bool recip = fabsf(piv) >= sfmin;
+                const float rinv = 1.0f / piv;
+                const int ncol = k + kb - j - 1;
+                for (int i = j + 1 + tid; i < n; i += 256) {
+                    float l = A[off(i, j, lda)];
+                    l = recip ? l * rinv : l / piv;
+                    A[off(i, j, lda)] = l;
+                    for (int c = 0; c < ncol; ++c) A[off(i, j + 1 + c, lda)] -= l * prow[c];
+                }
+            }
+            __syncthreads();
+        }
+        const int c1 = k + kb;
+        const int m = n - c1;
+        if (m <= 0) continue;
+        // U12 := L11^{-1} A12 (unit lower), one thread per column
+        for (int e = tid; e < kNB * kNB; e += 256) {
+            const int i = e % kNB, j = e / kNB;
+            l11[i][j] = (i < kb && j < kb) ? A[off(k + i, k + j, lda)] : 0.0f;
+        }
+        __syncthreads();
+        for (int c = c1 + tid; c < n; c += 256) {
+            float x[kNB];
+#pragma unroll
+            for (int i = 0; i < kNB; ++i) x[i] = i < kb ? A[off(k + i, c, lda)] : 0.0f;
+#pragma unroll
+            for (int i = 1; i < kNB; ++i) {
+                float v = x[i];
+#pragma unroll
+                for (int q = 0; q < i; ++q) v -= l11[i][q] * x[q];
+                x[i] = v;
+            }
+#pragma unroll
+            for (int i = 0; i < kNB; ++i)
+                if (i < kb) A[off(k + i, c, lda)] = x[i];
+        }
+        __syncthreads();
+        // A22 -= L21 U12
+        float* C = A + off(c1, c1, lda);
+        const float* L21 = A + off(c1, k, lda);
+        const float* U12 = A + off(k, c1, lda);
+        auto la = [&](int i, int kk) { return L21[off(i, kk, lda)]; };
+        auto rb = [&](int kk, int j) { return U12[off(kk, j, lda)]; };
+        const int tiles = (m + kT - 1) / kT;
+        for (int tj = 0; tj < tiles; ++tj)
+            for (int ti = 0; ti < tiles; ++ti)
+                tile_update<PREC, false>(C, lda, m, m, ti * kT, tj * kT, kb, la, rb, ts);
+    }
+}
+
+// ---------------------------------------------------------------- queue
+
+__device__ inline int next_item(unsigned* counter)
+{
+    __shared__ int item;
+    __syncthreads();
+    if (threadIdx.x == 0) item = static_cast<int>(atomicAdd(counter, 1u));
+    __syncthreads();
+    return item;
+}
+
+template <int PREC, bool LU>
+__global__ __launch_bounds__(256) void batched_kernel(BatchArgs a)
+{
+    for (;;) {
+        const int q = next_item(a.counter);
+        if (q >= a.batch) return;            // queue drained: every workgroup exits
+        const int id = a.order[q];
+        const int n = a.n[id];
+        if (n <= 0 || n > kMaxOrder) continue;
+        float* A = a.A + a.offsets[id];
+        const long long lda = a.lda[id];
+        int* info = a.info + id;
+        if (threadIdx.x == 0) *info = 0;
+        __syncthreads();
+        if (LU) lu_matrix<PREC>(A, lda, n, a.ipiv + a.ipiv_offsets[id], info);
+        else chol_matrix<PREC>(A, lda, n, info);
+    }
+}
+
+// Work order: ids sorted by decreasing order n (counting sort on n / 16),
+// and the queue counter reset (one workgroup).
+__global__ __launch_bounds__(1024) void order_kernel(const int* n, int batch, int* order,
+                                                     unsigned* counter)
+{
+    __shared__ int hist[kBuckets];
+    __shared__ int start[kBuckets];
+    for (int b = threadIdx.x; b < kBuckets; b += 1024) hist[b] = 0;
+    __syncthreads();
+    auto bucket = [&](int i) {
+        const int v = n[i] / 16;
+        return kBuckets - 1 - (v < 0 ? 0 : (v >= kBuckets ? kBuckets - 1 : v));
+    };
+    for (int i = threadIdx.x; i < batch; i += 1024) atomicAdd(&hist[bucket(i)], 1);
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        int s = 0;
+        for (int b = 0; b < kBuckets; ++b) {
+            start[b] = s;
+            s += hist[b];
+        }
+        *counter = 0u;
+    }
+    __syncthreads();
+    for (int i = threadIdx.x; i < batch; i += 1024) order[atomicAdd(&start[bucket(i)], 1)] = i;
+}
+
+// ---------------------------------------------------------------- solves
+
+// t := M^{-1} t for one matrix (t FP32 in global memory): LU: P, unit L,
+// U; Cholesky: L, L^T.  Column-oriented substitutions, two barriers per
+// column.
+template <bool LU>
+__device__ void tri_solve(const float* F, long long ldf, int n, const int* ipiv, float* t)
+{
+    const int tid = threadIdx.x;
+    if (LU) {
+        if (tid == 0) {
+            for (int j = 0; j < n; ++j) {
+                const int p = ipiv[j] - 1;
+                if (p != j && p >= 0 && p < n) {
+                    const float s = t[j];
+                    t[j] = t[p];
+                    t[p] = s;
+                }
+            }
+        }
+        __syncthreads();
+    }
+    for (int j = 0; j < n; ++j) {            // forward
+        if (!LU) {
+            if (tid == 0) t[j] = t[j] / F[off(j, j, ldf)];
+            __syncthreads();
+        }
+        const float tj = t[j];
+        for (int i = j + 1 + tid; i < n; i += 256) t[i] -= F[off(i, j, ldf)] * tj;
+        __syncthreads();
+    }
+    for (int j = n - 1; j >= 0; --j) {       // backward
+        if (tid == 0) t[j] = t[j] / F[off(j, j, ldf)];
+        __syncthreads();
+        const float tj = t[j];
+        for (int i = tid; i < j; i += 256)
+            t[i] -= (LU ? F[off(i, j, ldf)] : F[off(j, i, ldf)]) * tj;
+        __syncthreads();
+    }
+}
+
+struct SolveArgs {
+    const float* A;
+    const float* F;
+    const long long* offsets;
+    const int* n;
+    const int* lda;
+    const int* ipiv;
+    const long long* ipiv_offsets;
+    const double* B;
+    double* X;
+    const long long* vec_offsets;
+    float* t;                  // FP32 staging, vec_offsets layout
+    double* berr;
+    const int* info;
+    double tol;
+    int batch;
+    int sym;
+};
+
+// One refinement step per matrix (one workgroup per matrix):
+// r = b - A x (FP64, A FP32), berr, and unless converged x += M^{-1} r.
+// first: x = 0 on entry (x := M^{-1} b).
+template <bool LU>
+__global__ __launch_bounds__(256) void refine_step_kernel(SolveArgs a, int first)
+{
+    const int id = blockIdx.x;
+    if (id >= a.batch) return;
+    const int n = a.n[id];
+    if (n <= 0 || n > kMaxOrder || a.info[id] != 0) return;
+    if (!first && a.berr[id] <= a.tol) return;       // converged: skip
+    const float* A = a.A + a.offsets[id];
+    const float* F = a.F + a.offsets[id];
+    const long long lda = a.lda[id];
+    const double* b = a.B + a.vec_offsets[id];
+    double* x = a.X + a.vec_offsets[id];
+    float* t = a.t + a.vec_offsets[id];
+    const int tid = threadIdx.x;
+    __shared__ double red[256];
+    __shared__ double red2[256];
+    __shared__ double red3[256];
+    // r = b - A x; norms
+    double rmax = 0.0, xmax = 0.0, bmax = 0.0, amax = 0.0;
+    for (int i = tid; i < n; i += 256) {
+        double s = 0.0, as = 0.0;
+        if (!first) {
+            for (int c = 0; c < n; ++c) {
+                const double av = (!LU && c > i) ? A[off(c, i, lda)] : A[off(i, c, lda)];
+                s = fma(av, x[c], s);
+                as += fabs(av);
+            }
+        }
+        const double r = b[i] - s;
+        t[i] = static_cast<float>(r);
+        rmax = fmax(rmax, fabs(r));
+        xmax = fmax(xmax, first ? 0.0 : fabs(x[i]));
+        bmax = fmax(bmax, fabs(b[i]));
+        amax = fmax(amax, as);
+    }
+    red[tid] = rmax;
+    red2[tid] = xmax;
+    red3[tid] = amax;
+    __syncthreads();
+    for (int o = 128; o > 0; o >>= 1) {
+        if (tid < o) {
+            red[tid] = fmax(red[tid], red[tid + o]);
+            red2[tid] = fmax(red2[tid], red2[tid + o]);
+            red3[tid] = fmax(red3[tid], red3[tid + o]);
+        }
+        __syncthreads();
+    }
+    const double rn = red[0], xn = red2[0], an = red3[0];
+    __syncthreads();
+    red[tid] = bmax;
+    __syncthreads();
+    for (int o = 128; o > 0; o >>= 1) {
+        if (tid < o) red[tid] = fmax(red[tid], red[tid + o]);
+        __syncthreads();
+    }
+    const double bn = red[0];
+    if (!first) {
+        const double den = an * xn + bn;
+        const double e = den > 0.0 ? rn / den : (rn > 0.0 ? INFINITY : 0.0);
+        if (tid == 0) a.berr[id] = e;
+        if (e <= a.tol) return;
+    } else if (tid == 0) {
+        a.berr[id] = INFINITY;
+    }
+    __syncthreads();
+    tri_solve<LU>(F, lda, n, LU ? a.ipiv + a.ipiv_offsets[id] : nullptr, t);
+    for (int i = tid; i < n; i += 256) x[i] = (first ? 0.0 : x[i]) + static_cast<double>(t[i]);
+}
+
+// Final backward error of every matrix (after the last update).
+template <bool LU>
+__global__ __launch_bounds__(256) void berr_kernel(SolveArgs a)
+{
+    const int id = blockIdx.x;
+    if (id >= a.batch) return;
+    const int n = a.n[id];
+    if (n <= 0 || n > kMaxOrder || a.info[id] != 0) return;
+    if (a.berr[id] <= a.tol) return;
+    const float* A = a.A + a.offsets[id];
+    const long long lda = a.lda[id];
+    const double* b = a.B + a.vec_offsets[id];
+    const double* x = a.X + a.vec_offsets[id];
+    const int tid = threadIdx.x;
+    __shared__ double r1[256], r2[256], r3[256], r4[256];
+    double rmax = 0.0, xmax = 0.0, bmax = 0.0, amax = 0.0;
+    for (int i = tid; i < n; i += 256) {
+        double s = 0.0, as = 0.0;
+        for (int c = 0; c < n; ++c) {
+            const double av = (!LU && c > i) ? A[off(c, i, lda)] : A[off(i, c, lda)];
+            s = fma(av, x[c], s);
+            as += fabs(av);
+        }
+        rmax = fmax(rmax, fabs(b[i] - s));
+        xmax = fmax(xmax, fabs(x[i]));
+        bmax = fmax(bmax, fabs(b[i]));
+        amax = fmax(amax, as);
+    }
+    r1[tid] = rmax;
+    r2[tid] = xmax;
+    r3[tid] = bmax;
+    r4[tid] = amax;
+    __syncthreads();
+    for (int o = 128; o > 0; o >>= 1) {
+        if (tid < o) {
+            r1[tid] = fmax(r1[tid], r1[tid + o]);
+            r2[tid] = fmax(r2[tid], r2[tid + o]);
+            r3[tid] = fmax(r3[tid], r3[tid + o]);
+            r4[tid] = fmax(r4[tid], r4[tid + o]);
+        }
+        __syncthreads();
+    }
+    if (tid == 0) {
+        const double den = r4[0] * r2[0] + r3[0];
+        a.berr[id] = den > 0.0 ? r1[0] / den : (r1[0] > 0.0 ? INFINITY : 0.0);
+    }
+}
+
+// F = A (copy of every matrix; lower triangle for SPD)
+__global__ __launch_bounds__(256) void copy_batch_kernel(const float* A, float* F,
+                                                         const long long* offsets, const int* n,
+                                                         const int* lda, int batch, int sym)
+{
+    const int id = blockIdx.x;
+    if (id >= batch) return;
+    const int nn = n[id];
+    if (nn <= 0 || nn > kMaxOrder) return;
+    const long long ld = lda[id];
+    const float* a = A + offsets[id];
+    float* f = F + offsets[id];
+    for (int c = 0; c < nn; ++c)
+        for (int i = threadIdx.x; i < nn; i += 256)
+            if (!sym || i >= c) f[off(i, c, ld)] = a[off(i, c, ld)];
+}
+
+// ---------------------------------------------------------------- host
+
+struct BatchScratch {
+    int* order = nullptr;
+    unsigned* counter = nullptr;
+    float* t = nullptr;
+};
+
+bool batch_scratch(mifact_handle h, int batch, size_t t_elems, BatchScratch* s)
+{
+    const size_t need = 256 + sizeof(int) * static_cast<size_t>(batch) + 256
+                      + sizeof(float) * t_elems + 256;
+    if (h->batch_ws == nullptr || h->batch_ws_bytes < need) {
+        if (h->batch_ws != nullptr) {
+            if (hipStreamSynchronize(h->stream) != hipSuccess) return false;
+            (void)hipFree(h->batch_ws);
+            h->batch_ws = nullptr;
+            h->batch_ws_bytes = 0;
+        }
+        if (hipMalloc(&h->batch_ws, need) != hipSuccess) {
+            h->batch_ws = nullptr;
+            report_status(h, mifact_status_allocation_failed, "batch scratch allocation failed");
+            return false;
+        }
+        h->batch_ws_bytes = need;
+    }
+    char* base = static_cast<char*>(h->batch_ws);
+    s->counter = reinterpret_cast<unsigned*>(base);
+    s->order = reinterpret_cast<int*>(base + 256);
+    const size_t o = 256 + ((sizeof(int) * static_cast<size_t>(batch) + 255) & ~size_t(255));
+    s->t = reinterpret_cast<float*>(base + o);
+    return true;
+}
+
+void launch_batched(mifact_handle h, bool lu, LowPrecision prec, int batch, float* A,
+                    const long long* offsets, const int* n, const int* lda, int* ipiv,
+                    const long long* ipiv_offsets, int* info, const BatchScratch& s)
+{
+    hipLaunchKernelGGL(order_kernel, dim3(1), dim3(1024), 0, h->stream, n, batch, s.order,
+                       s.counter);
+    MIFACT_LAUNCHED(h);
+    BatchArgs a{A, offsets, n, lda, ipiv, ipiv_offsets, info, s.order, s.counter, batch};
+    const int grid = imax(1, imin(batch, h->num_cu * 4));
+    StageScope scope(h, kStageBatched, 0.0, 0.0);
+    auto go = [&](auto kern) {
+        hipLaunchKernelGGL(kern, dim3(grid), dim3(256), 0, h->stream, a);
+    };
+    if (lu) {
+        if (prec == kPrecBF16) go(batched_kernel<kPrecBF16, true>);
+        else if (prec == kPrecFP16) go(batched_kernel<kPrecFP16, true>);
+        else go(batched_kernel<kPrecFP32, true>);
+    } else {
+        if (prec == kPrecBF16) go(batched_kernel<kPrecBF16, false>);
+        else if (prec == kPrecFP16) go(batched_kernel<kPrecFP16, false>);
+        else go(batched_kernel<kPrecFP32, false>);
+    }
+    MIFACT_LAUNCHED(h);
+}
+
+bool valid_prec(int p) { return p == kPrecFP32 || p == kPrecBF16 || p == kPrecFP16; }
+
+// The batched LU kernel is switched off: its first GPU run ended in a memory
+// aperture fault whose cause has not been identified (every index of
+// lu_matrix checks out against the matrix bounds on reading).  Until it is,
+// the LU entry points report mifact_status_invalid_value without launching.
+constexpr bool kBatchedLuEnabled = false;
+
+void batched_entry(mifact_handle h, bool lu, int prec, int batch, float* A,
+                   const long long* offsets, const int* n, const int* lda, int* ipiv,
+                   const long long* ipiv_offsets, int* info)
+{
+    if (h == nullptr) {
+        report_status(nullptr, mifact_status_invalid_handle, "batched factorization");
+        return;
+    }
+    if (batch <= 0) return;
+    if (lu && !kBatchedLuEnabled) {
+        report_status(h, mifact_status_invalid_value, "batched LU is not available in this build");
+        return;
+    }
+    if (!valid_prec(prec) || A == nullptr || offsets == nullptr || n == nullptr
+        || lda == nullptr || info == nullptr || (lu && (ipiv == nullptr || ipiv_offsets == nullptr))) {
+        report_status(h, mifact_status_invalid_value, "batched factorization: arguments");
+        return;
+    }
+    BatchScratch s;
+    if (!batch_scratch(h, batch, 0, &s)) return;
+    launch_batched(h, lu, static_cast<LowPrecision>(prec), batch, A, offsets, n, lda, ipiv,
+                   ipiv_offsets, info, s);
+}
+
+}  // namespace
+}  // namespace mifact
+
+using namespace mifact;
+
+extern "C" {
+
+void mifact_potrf_batched(mifact_handle handle, int batch, float* A, const long long* offsets,
+                          const int* n, const int* lda, int* info)
+{
+    batched_entry(handle, false, kPrecFP32, batch, A, offsets, n, lda, nullptr, nullptr, info);
+}
+
+void mifact_getrf_batched(mifact_handle handle, int batch, float* A, const long long* offsets,
+                          const int* n, const int* lda, int* ipiv, const long long* ipiv_offsets,
+                          int* info)
+{
+    batched_entry(handle, true, kPrecFP32, batch, A, offsets, n, lda, ipiv, ipiv_offsets, info);
+}
+
+void mifact_potrf_batched_ex(mifact_handle handle, mifact_precision precision, int batch,
+                             float* A, const long long* offsets, const int* n, const int* lda,
+                             int* info)
+{
+    batched_entry(handle, false, precision, batch, A, offsets, n, lda, nullptr, nullptr, info);
+}
+
+void mifact_getrf_batched_ex(mifact_handle handle, mifact_precision precision, int batch,
+                             float* A, const long long* offsets, const int* n, const int* lda,
+                             int* ipiv, const long long* ipiv_offsets, int* info)
+{
+    batched_entry(handle, true, precision, batch, A, offsets, n, lda, ipiv, ipiv_offsets, info);
+}
+
+mifact_status mifact_gesv_batched_refine(mifact_handle handle, int sym,
+                                         mifact_precision precision, int batch, const float* A,
+                                         float* F, const long long* offsets, const int* n,
+                                         const int* lda, int* ipiv, const long long* ipiv_offsets,
+                                         const double* B, double* X,
+                                         const long long* vec_offsets, long long vec_elems,
+                                         int iterations, double tolerance, double* berr,
+                                         int* info)
+{
+    mifact_handle h = handle;
+    if (h == nullptr) {
+        report_status(nullptr, mifact_status_invalid_handle, "mifact_gesv_batched_refine");
+        return mifact_status_invalid_handle;
+    }
+    if (batch <= 0) return mifact_status_success;
+    const bool lu = sym == 0;
+    if (lu && !kBatchedLuEnabled) {
+        report_status(h, mifact_status_invalid_value, "batched LU is not available in this build");
+        return mifact_status_invalid_value;
+    }
+    if (!valid_prec(precision) || iterations < 0 || vec_elems <= 0 || A == nullptr
+        || F == nullptr || offsets == nullptr || n == nullptr || lda == nullptr || B == nullptr
+        || X == nullptr || vec_offsets == nullptr || berr == nullptr || info == nullptr
+        || (lu && (ipiv == nullptr || ipiv_offsets == nullptr))) {
+        report_status(h, mifact_status_invalid_value, "mifact_gesv_batched_refine: arguments");
+        return mifact_status_invalid_value;
+    }
+    const mifact_status before = h->last_error;
+    h->last_error = mifact_status_success;
+    auto finish = [&]() {
+        mifact_status s = h->last_error;
+        if (before != mifact_status_success) h->last_error = before;
+        return s;
+    };
+    BatchScratch s;
+    if (!batch_scratch(h, batch, static_cast<size_t>(vec_elems), &s)) return finish();
+    hipLaunchKernelGGL(copy_batch_kernel, dim3(batch), dim3(256), 0, h->stream, A, F, offsets, n,
+                       lda, batch, sym ? 1 : 0);
+    MIFACT_HIP_RET(h, hipGetLastError(), finish());
+    launch_batched(h, lu, static_cast<LowPrecision>(precision), batch, F, offsets, n, lda, ipiv,
+                   ipiv_offsets, info, s);
+    if (h->last_error != mifact_status_success) return finish();
+
+    // The refinement loop (first solve, `iterations` steps, final backward
+    // errors) is captured once on a private stream and replayed on the
+    // handle's stream as one graph launch.
+    if (h->capture_stream == nullptr) {
+        MIFACT_HIP_RET(h, hipStreamCreateWithFlags(&h->capture_stream, hipStreamNonBlocking),
+                       finish());
+    }
+    SolveArgs sa{A, F, offsets, n, lda, ipiv, ipiv_offsets, B, X, vec_offsets, s.t, berr, info,
+                 tolerance, batch, sym};
+    hipStream_t cs = h->capture_stream;
+    MIFACT_HIP_RET(h, hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal), finish());
+    for (int it = 0; it <= iterations; ++it) {
+        if (lu) hipLaunchKernelGGL(refine_step_kernel<true>, dim3(batch), dim3(256), 0, cs, sa, it == 0);
+        else hipLaunchKernelGGL(refine_step_kernel<false>, dim3(batch), dim3(256), 0, cs, sa, it == 0);
+    }
+    if (lu) hipLaunchKernelGGL(berr_kernel<true>, dim3(batch), dim3(256), 0, cs, sa);
+    else hipLaunchKernelGGL(berr_kernel<false>, dim3(batch), dim3(256), 0, cs, sa);
+    hipGraph_t graph = nullptr;
+    const hipError_t ec = hipStreamEndCapture(cs, &graph);
+    MIFACT_HIP_RET(h, ec, finish());
+    hipGraphExec_t exec = nullptr;
+    const hipError_t ei = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(graph);
+    MIFACT_HIP_RET(h, ei, finish());
+    {
+        StageScope scope(h, kStageBatched, 0.0, 0.0);
+        const hipError_t el = hipGraphLaunch(exec, h->stream);
+        if (el != hipSuccess) {
+            (void)hipGraphExecDestroy(exec);
+            MIFACT_HIP_RET(h, el, finish());
+        }
+    }
+    // the executable graph may be destroyed once its launch is enqueued
+    (void)hipGraphExecDestroy(exec);
+    return finish();
+}
+
+}  // extern "C"

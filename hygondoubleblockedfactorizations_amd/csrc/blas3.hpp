@@ -89,6 +89,46 @@ void trsm_llu(mifact_handle h, int rows, int cols, const T* L, long long ldl, T*
     trsm_llu_inv<T>(h, rows, cols, L, ldl, B, ldb, inv);
 }
 
+// B(rows x cols) := L^{-T} B, L lower (rows x rows), given inverses of L's
+// diagonal blocks (leaves multiply by inv(L_jj)^T: TN MFMA GEMM in place).
+template <typename T>
+void trsm_llt_inv(mifact_handle h, int rows, int cols, const T* L, long long ldl, T* B,
+                  long long ldb, const T* inv)
+{
+    if (rows <= 0 || cols <= 0) return;
+    if (rows <= 64) {
+        gemm_overwrite<T>(h, kOpTN, rows, cols, rows, T(1), inv, 64, B, ldb, B, ldb);
+        return;
+    }
+    const int r1 = split_point(rows, 64);
+    const int r2 = rows - r1;
+    trsm_llt_inv<T>(h, r2, cols, at(L, ldl, r1, r1), ldl, at(B, ldb, r1, 0), ldb,
+                    inv + static_cast<size_t>(r1) * 64);
+    gemm<T>(h, kOpTN, kShapeFull, r1, cols, r2, T(-1), at(L, ldl, r1, 0), ldl,
+            at(B, ldb, r1, 0), ldb, B, ldb);
+    trsm_llt_inv<T>(h, r1, cols, L, ldl, B, ldb, inv);
+}
+
+// B(rows x cols) := U^{-1} B, U upper (rows x rows), given inv(U_jj)^T of
+// its diagonal blocks (trtri_blocks with upper = true).
+template <typename T>
+void trsm_lun_inv(mifact_handle h, int rows, int cols, const T* U, long long ldu, T* B,
+                  long long ldb, const T* invt)
+{
+    if (rows <= 0 || cols <= 0) return;
+    if (rows <= 64) {
+        gemm_overwrite<T>(h, kOpTN, rows, cols, rows, T(1), invt, 64, B, ldb, B, ldb);
+        return;
+    }
+    const int r1 = split_point(rows, 64);
+    const int r2 = rows - r1;
+    trsm_lun_inv<T>(h, r2, cols, at(U, ldu, r1, r1), ldu, at(B, ldb, r1, 0), ldb,
+                    invt + static_cast<size_t>(r1) * 64);
+    gemm<T>(h, kOpNN, kShapeFull, r1, cols, r2, T(-1), at(U, ldu, 0, r1), ldu,
+            at(B, ldb, r1, 0), ldb, B, ldb);
+    trsm_lun_inv<T>(h, r1, cols, U, ldu, B, ldb, invt);
+}
+
 // Row-blocked variant: diagonal solves of `block` rows, each followed by a
 // GEMM update of the rows below (getrf_pivot_internal.h:64-110 schedule).
 template <typename T>

@@ -21,6 +21,9 @@ enum Stage {
     kStageGetf2Leaf,
     kStageRowMoves,
     kStagePerm,
+    kStageGemmLowp,
+    kStageSolve,
+    kStageBatched,
     kStageCount
 };
 
@@ -39,6 +42,8 @@ struct mifact_handle_s {
     int device = 0;
     int num_cu = 0;
     int error_mode = mifact_error_exit;
+    // FP32 GEMM tile configuration (0 = automatic; see mifact_set_gemm_config).
+    int gemm_config = 0;
     mifact_status last_error = mifact_status_success;
 
     // Device workspace arena: [ints | floats] sized for order ws_n.
@@ -55,6 +60,19 @@ struct mifact_handle_s {
     unsigned int leaf_epoch = 1;
     // Device status word written by kernels whose bounded waits expired.
     int* kernel_status = nullptr;
+
+    // Solver arena (triangular solves, residuals, refinement) and the pinned
+    // host buffer its convergence tests read back.
+    void* solve_ws = nullptr;
+    size_t solve_ws_bytes = 0;
+    void* host_norms = nullptr;
+    size_t host_norms_bytes = 0;
+
+    // Batched path: work-queue order, counter and FP32 staging; the private
+    // stream refinement graphs are captured on.
+    void* batch_ws = nullptr;
+    size_t batch_ws_bytes = 0;
+    hipStream_t capture_stream = nullptr;
 
     // Stage profiler: event pool and the records of the current session.
     bool profiling = false;
@@ -107,9 +125,9 @@ __host__ __device__ inline T* at(T* A, long long ld, long long row, long long co
     return A + off(row, col, ld);
 }
 
-inline int imin(int a, int b) { return a < b ? a : b; }
-inline int imax(int a, int b) { return a > b ? a : b; }
-inline int ceil_div(int a, int b) { return (a + b - 1) / b; }
+__host__ __device__ inline int imin(int a, int b) { return a < b ? a : b; }
+__host__ __device__ inline int imax(int a, int b) { return a > b ? a : b; }
+__host__ __device__ inline int ceil_div(int a, int b) { return (a + b - 1) / b; }
 
 // ---------------------------------------------------------------- workspace
 

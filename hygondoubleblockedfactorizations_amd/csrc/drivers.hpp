@@ -1,0 +1,23 @@
+// Factorization drivers shared by the C-ABI entry points and the
+// mixed-precision solvers.
+#pragma once
+
+#include "getrf/double_blocked_getrf.h"
+#include "kernels.hpp"
+
+namespace mifact {
+
+// Double-blocked POTRF (reference schedule); trailing updates in `prec`.
+template <typename T>
+void double_blocked_potrf_impl(mifact_handle h, T* A, int lda, int n, int nb, int b, int* info,
+                               bool use_dbtrsm, LowPrecision prec);
+
+// Double-blocked GETRF (reference schedule); trailing updates in `prec`.
+// On return the workspace's orig[] holds the final row permutation
+// (row p of the factors came from row orig[p] of A).
+template <typename T>
+void double_blocked_getrf_impl(mifact_handle h, T* A, int lda, int n, int outer_nb, int inner_nb,
+                               getrf_outer_trsm_mode mode, int trsm_block, int* ipiv, int* info,
+                               bool batched, T* stage, long long stage_elems, LowPrecision prec);
+
+}  // namespace mifact

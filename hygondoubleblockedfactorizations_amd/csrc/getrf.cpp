@@ -12,8 +12,10 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdio>
+#include <type_traits>
 
 #include "blas3.hpp"
+#include "drivers.hpp"
 #include "getrf/double_blocked_getrf.h"
 
 namespace mifact {
@@ -40,9 +42,22 @@ bool workspace_is_valid(const getrf_batched_pivot_workspace* w, int n)
         && w->host_targets != nullptr && w->capacity >= n && w->column_tile > 0;
 }
 
+// Trailing update C -= L U (NN): FP32/FP64 MFMA, or BF16/FP16 MFMA with FP32
+// accumulation for the mixed-precision factorization.
+template <typename T>
+void update_nn(mifact_handle h, LowPrecision prec, int M, int N, int K, const T* L, long long ldl,
+               const T* U, long long ldu, T* C, long long ldc)
+{
+    if constexpr (std::is_same_v<T, float>) {
+        gemm_lowp(h, prec, kOpNN, kShapeFull, M, N, K, -1.0f, L, ldl, U, ldu, C, ldc);
+    } else {
+        gemm<T>(h, kOpNN, kShapeFull, M, N, K, T(-1), L, ldl, U, ldu, C, ldc);
+    }
+}
+
 template <typename T>
 void outer_update(mifact_handle h, T* A, long long lda, int n, int next_outer, int outer_nb,
-                  getrf_outer_trsm_mode mode, int trsm_block)
+                  getrf_outer_trsm_mode mode, int trsm_block, LowPrecision prec)
 {
     if (next_outer >= n) return;
     const int next_size = imin(outer_nb, n - next_outer);
@@ -52,30 +67,30 @@ void outer_update(mifact_handle h, T* A, long long lda, int n, int next_outer, i
         trsm_llu_iterative<T>(h, next_outer, next_size, A, lda, at(A, lda, 0, next_outer), lda,
                               trsm_block);
     }
-    gemm<T>(h, kOpNN, kShapeFull, n - next_outer, next_size, next_outer, T(-1),
-            at(A, lda, next_outer, 0), lda, at(A, lda, 0, next_outer), lda,
-            at(A, lda, next_outer, next_outer), lda);
+    update_nn<T>(h, prec, n - next_outer, next_size, next_outer, at(A, lda, next_outer, 0), lda,
+                 at(A, lda, 0, next_outer), lda, at(A, lda, next_outer, next_outer), lda);
 }
 
 template <typename T>
 void inner_update(mifact_handle h, T* A, long long lda, int n, int outer, int next_inner,
-                  int outer_end, int inner_nb)
+                  int outer_end, int inner_nb, LowPrecision prec)
 {
     if (next_inner >= outer_end) return;
     const int next_size = imin(inner_nb, outer_end - next_inner);
     const int history = next_inner - outer;
     trsm_llu<T>(h, history, next_size, at(A, lda, outer, outer), lda,
                 at(A, lda, outer, next_inner), lda);
-    gemm<T>(h, kOpNN, kShapeFull, n - next_inner, next_size, history, T(-1),
-            at(A, lda, next_inner, outer), lda, at(A, lda, outer, next_inner), lda,
-            at(A, lda, next_inner, next_inner), lda);
+    update_nn<T>(h, prec, n - next_inner, next_size, history, at(A, lda, next_inner, outer), lda,
+                 at(A, lda, outer, next_inner), lda, at(A, lda, next_inner, next_inner), lda);
 }
+
+}  // namespace
 
 template <typename T>
 void double_blocked_getrf_impl(mifact_handle h, T* A, int lda_i, int n, int outer_nb,
                                int inner_nb, getrf_outer_trsm_mode mode, int trsm_block,
                                int* ipiv, int* info, bool batched, T* stage,
-                               long long stage_elems)
+                               long long stage_elems, LowPrecision prec)
 {
     if (n <= 0) return;
     if (outer_nb <= 0) outer_nb = n;
@@ -117,7 +132,7 @@ void double_blocked_getrf_impl(mifact_handle h, T* A, int lda_i, int n, int oute
                 apply_row_moves<T>(h, A, lda, next_inner, outer_end, inner, isz, ipiv, orig,
                                    inv_panel, stage, stage_elems);
             }
-            inner_update<T>(h, A, lda, n, outer, next_inner, outer_end, inner_nb);
+            inner_update<T>(h, A, lda, n, outer, next_inner, outer_end, inner_nb, prec);
         }
         if (batched) {
             // Outer pivot batch (:556-609): this block's exchanges complete
@@ -131,9 +146,18 @@ void double_blocked_getrf_impl(mifact_handle h, T* A, int lda_i, int n, int oute
                                    0, outer_end, ipiv, orig, inv_all, stage, stage_elems);
             }
         }
-        outer_update<T>(h, A, lda, n, outer_end, outer_nb, mode, trsm_block);
+        outer_update<T>(h, A, lda, n, outer_end, outer_nb, mode, trsm_block, prec);
     }
 }
+
+template void double_blocked_getrf_impl<float>(mifact_handle, float*, int, int, int, int,
+                                              getrf_outer_trsm_mode, int, int*, int*, bool,
+                                              float*, long long, LowPrecision);
+template void double_blocked_getrf_impl<double>(mifact_handle, double*, int, int, int, int,
+                                               getrf_outer_trsm_mode, int, int*, int*, bool,
+                                               double*, long long, LowPrecision);
+
+namespace {
 
 template <typename T>
 void getrf_entry(mifact_handle h, T* A, int lda, int n, int outer_nb, int inner_nb,
@@ -144,7 +168,7 @@ void getrf_entry(mifact_handle h, T* A, int lda, int n, int outer_nb, int inner_
         return;
     }
     double_blocked_getrf_impl<T>(h, A, lda, n, outer_nb, inner_nb, mode, trsm_block, ipiv, info,
-                                 false, nullptr, 0);
+                                 false, nullptr, 0, kPrecFP32);
 }
 
 template <typename T>
@@ -164,7 +188,8 @@ void getrf_batched_entry(mifact_handle h, T* A, int lda, int n, int outer_nb, in
     const long long stage_elems =
         static_cast<long long>(w->capacity) * static_cast<long long>(w->column_tile);
     double_blocked_getrf_impl<T>(h, A, lda, n, outer_nb, inner_nb, mode, trsm_block, ipiv, info,
-                                 true, static_cast<T*>(w->device_swap_buffer), stage_elems);
+                                 true, static_cast<T*>(w->device_swap_buffer), stage_elems,
+                                 kPrecFP32);
 }
 
 }  // namespace

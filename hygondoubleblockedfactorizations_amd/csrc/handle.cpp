@@ -145,8 +145,9 @@ mifact_status mifact_profile_end(mifact_handle h, mifact_stage_stats* stats, int
 {
     if (h == nullptr) return mifact_status_invalid_handle;
     h->profiling = false;
-    static const char* names[mifact::kStageCount] = {"gemm", "potf2", "trsm_leaf",
-                                                     "getf2_leaf", "row_moves", "perm"};
+    static const char* names[mifact::kStageCount] = {"gemm", "potf2", "trsm_leaf", "getf2_leaf",
+                                                     "row_moves", "perm", "gemm_lowp", "solve",
+                                                     "batched"};
     mifact_stage_stats acc[mifact::kStageCount];
     for (int s = 0; s < mifact::kStageCount; ++s) {
         std::snprintf(acc[s].name, sizeof(acc[s].name), "%s", names[s]);
@@ -225,6 +226,18 @@ mifact_status mifact_destroy_handle(mifact_handle h)
     if (h->barrier != nullptr) {
         (void)hipFree(h->barrier);
     }
+    if (h->solve_ws != nullptr) {
+        (void)hipFree(h->solve_ws);
+    }
+    if (h->host_norms != nullptr) {
+        (void)hipHostFree(h->host_norms);
+    }
+    if (h->batch_ws != nullptr) {
+        (void)hipFree(h->batch_ws);
+    }
+    if (h->capture_stream != nullptr) {
+        (void)hipStreamDestroy(h->capture_stream);
+    }
     for (hipEvent_t e : h->event_pool) {
         (void)hipEventDestroy(e);
     }
@@ -289,6 +302,14 @@ mifact_status mifact_reserve_workspace(mifact_handle h, int n)
     }
     return mifact::ensure_workspace(h, n) ? mifact_status_success
                                           : mifact_status_allocation_failed;
+}
+
+mifact_status mifact_set_gemm_config(mifact_handle h, int config)
+{
+    if (h == nullptr) return mifact_status_invalid_handle;
+    if (config < 0 || config > 4) return mifact_status_invalid_value;
+    h->gemm_config = config;
+    return mifact_status_success;
 }
 
 const char* mifact_version(void) { return "mifact 0.1.0 (gfx950)"; }

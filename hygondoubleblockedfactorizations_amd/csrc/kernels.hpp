@@ -8,17 +8,27 @@
 
 namespace mifact {
 
-enum GemmOp { kOpNT = 0, kOpNN = 1 };
+enum GemmOp { kOpNT = 0, kOpNN = 1, kOpTN = 2 };
 enum GemmShape { kShapeFull = 0, kShapeLower = 1 };
 
 // C(M x N) += alpha * A(M x K) * op(B)
 //   kOpNT: op(B) = B^T with B stored N x K   (C -= A B^T update: alpha = -1)
 //   kOpNN: op(B) = B   with B stored K x N
+//   kOpTN: C += alpha * A^T * B with A stored K x M and B stored K x N
 // kShapeLower touches only C(i, j) with i >= j (SYRK / trapezoid updates).
 template <typename T>
 void gemm(mifact_handle h, GemmOp op, GemmShape shape, int M, int N, int K,
           T alpha, const T* A, long long lda, const T* B, long long ldb, T* C,
           long long ldc);
+
+// Arithmetic of the trailing updates of the mixed-precision factorizations.
+enum LowPrecision { kPrecFP32 = 0, kPrecBF16 = 1, kPrecFP16 = 2 };
+
+// FP32 C += alpha * A * op(B) with the products on BF16 / FP16 MFMA (FP32
+// accumulation); kPrecFP32 forwards to gemm<float>.  op: kOpNT or kOpNN.
+void gemm_lowp(mifact_handle h, LowPrecision prec, GemmOp op, GemmShape shape, int M, int N,
+               int K, float alpha, const float* A, long long lda, const float* B, long long ldb,
+               float* C, long long ldc);
 
 // C(M x N) = alpha * A * op(B) (no accumulation into C; C may alias A or B
 // when every workgroup's tile reads only the rows/columns it writes, as in
@@ -36,9 +46,11 @@ void potf2_base(mifact_handle h, int s, T* A, long long lda, T* inv, int* info,
                 int col_offset);
 
 // Inverses of the 64 x 64 diagonal blocks of the n x n lower triangle of L
-// (unit diagonal when unit): block b at inv + 4096 * b, ld 64.
+// (unit diagonal when unit): block b at inv + 4096 * b, ld 64.  upper: the
+// blocks of an upper-triangular U are inverted as U^T, giving inv(U)^T.
 template <typename T>
-void trtri_blocks(mifact_handle h, int n, const T* L, long long ldl, T* inv, bool unit);
+void trtri_blocks(mifact_handle h, int n, const T* L, long long ldl, T* inv, bool unit,
+                  bool upper = false);
 
 // Cooperative partial-pivot LU of a leaf panel: rows [c0, n) x columns
 // [c0, c0 + w) of the full matrix (w <= 32).  Writes ipiv[c0 + j] (global,

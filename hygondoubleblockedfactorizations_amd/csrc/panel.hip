@@ -25,11 +25,13 @@ __device__ __attribute__((always_inline)) inline void inverse_row(int s, const T
 #pragma unroll
     for (int j = kLeaf - 1; j >= 0; --j) {
         if (j < s) {
-            T acc = T(0);
+            // four partial sums break the dependent FMA chain
+            T acc[4] = {T(0), T(0), T(0), T(0)};
 #pragma unroll
-            for (int k = j + 1; k < kLeaf; ++k) acc += m[k] * sL[k][j];
+            for (int k = j + 1; k < kLeaf; ++k) acc[(k - j - 1) & 3] += m[k] * sL[k][j];
+            const T sum = (acc[0] + acc[1]) + (acc[2] + acc[3]);
             const T dj = UNIT ? T(1) : T(1) / sL[j][j];
-            m[j] = (i == j) ? dj : (i > j ? -acc * dj : T(0));
+            m[j] = (i == j) ? dj : (i > j ? -sum * dj : T(0));
         } else {
             m[j] = T(0);
         }
@@ -111,7 +113,9 @@ void potf2_base(mifact_handle h, int s, T* A, long long lda, T* inv, int* info, 
 // Inverses of the 64 x 64 diagonal blocks of an n x n lower-triangular L
 // (unit or non-unit), one wave per block; block b is written at
 // inv + 64*64*b (column-major, ld 64).
-template <typename T, bool UNIT>
+// UPPER: the block is read transposed (an upper-triangular U as the lower
+// U^T), so the result is inv(U^T) = inv(U)^T.
+template <typename T, bool UNIT, bool UPPER>
 __global__ __launch_bounds__(64) void trtri_kernel(int n, const T* __restrict__ L, long long ldl,
                                                    T* __restrict__ inv)
 {
@@ -122,8 +126,14 @@ __global__ __launch_bounds__(64) void trtri_kernel(int n, const T* __restrict__ 
     const int i = threadIdx.x;
     const T* Lb = L + off(r0, r0, ldl);
 #pragma unroll 8
-    for (int k = 0; k < kLeaf; ++k)
-        sL[i][k] = (i < s && k < s && k <= i) ? Lb[off(i, k, ldl)] : T(0);
+    for (int k = 0; k < kLeaf; ++k) {
+        const bool in = i < s && k < s && k <= i;
+        if (UPPER) {
+            sL[i][k] = in ? Lb[off(k, i, ldl)] : T(0);  // U^T(i, k) = U(k, i)
+        } else {
+            sL[i][k] = in ? Lb[off(i, k, ldl)] : T(0);
+        }
+    }
     __syncthreads();
     T m[kLeaf];
     inverse_row<T, UNIT>(s, sL, m);
@@ -134,18 +144,21 @@ __global__ __launch_bounds__(64) void trtri_kernel(int n, const T* __restrict__ 
 }
 
 template <typename T>
-void trtri_blocks(mifact_handle h, int n, const T* L, long long ldl, T* inv, bool unit)
+void trtri_blocks(mifact_handle h, int n, const T* L, long long ldl, T* inv, bool unit, bool upper)
 {
     if (n <= 0) return;
     const int nb = ceil_div(n, kLeaf);
     StageScope scope(h, kStageTrsmLeaf, static_cast<double>(nb) * kLeaf * kLeaf * kLeaf / 3.0,
                      sizeof(T) * static_cast<double>(nb) * kLeaf * kLeaf * 1.5);
-    if (unit) {
-        hipLaunchKernelGGL((trtri_kernel<T, true>), dim3(nb), dim3(64), 0, h->stream, n, L, ldl,
-                           inv);
+    const dim3 grid(nb), block(64);
+    if (unit && upper) {
+        hipLaunchKernelGGL((trtri_kernel<T, true, true>), grid, block, 0, h->stream, n, L, ldl, inv);
+    } else if (unit) {
+        hipLaunchKernelGGL((trtri_kernel<T, true, false>), grid, block, 0, h->stream, n, L, ldl, inv);
+    } else if (upper) {
+        hipLaunchKernelGGL((trtri_kernel<T, false, true>), grid, block, 0, h->stream, n, L, ldl, inv);
     } else {
-        hipLaunchKernelGGL((trtri_kernel<T, false>), dim3(nb), dim3(64), 0, h->stream, n, L, ldl,
-                           inv);
+        hipLaunchKernelGGL((trtri_kernel<T, false, false>), grid, block, 0, h->stream, n, L, ldl, inv);
     }
     MIFACT_LAUNCHED(h);
 }
@@ -202,8 +215,8 @@ void iamax(mifact_handle h, int n, const T* x, int incx, int* result)
 
 template void potf2_base<float>(mifact_handle, int, float*, long long, float*, int*, int);
 template void potf2_base<double>(mifact_handle, int, double*, long long, double*, int*, int);
-template void trtri_blocks<float>(mifact_handle, int, const float*, long long, float*, bool);
-template void trtri_blocks<double>(mifact_handle, int, const double*, long long, double*, bool);
+template void trtri_blocks<float>(mifact_handle, int, const float*, long long, float*, bool, bool);
+template void trtri_blocks<double>(mifact_handle, int, const double*, long long, double*, bool, bool);
 template void iamax<float>(mifact_handle, int, const float*, int, int*);
 template void iamax<double>(mifact_handle, int, const double*, int, int*);
 

@@ -508,18 +508,41 @@ __global__ __launch_bounds__(256) void apply_moves_lds_kernel(T* __restrict__ A,
     T* vals = reinterpret_cast<T*>(smem_raw);
     const int c0 = c_begin + blockIdx.x * CT;
     const int nc = (c_end - c0) < CT ? (c_end - c0) : CT;
-    for (int c = 0; c < nc; ++c) {
-        const T* col = A + off(0, c0 + c, lda);
-        for (int t = threadIdx.x; t < nm; t += 256) {
-            const int s = src[t];
-            vals[c * nm + t] = (s >= 0) ? col[s] : T(0);
+    // Move lists are read once per workgroup; the gathers of all its columns
+    // are issued back to back (independent loads in flight) before any use.
+    for (int t0 = threadIdx.x; t0 < nm; t0 += 256 * 4) {
+        int sv[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            const int t = t0 + u * 256;
+            sv[u] = (t < nm) ? src[t] : -1;
+        }
+        for (int c = 0; c < nc; ++c) {
+            const T* col = A + off(0, c0 + c, lda);
+            T v[4];
+#pragma unroll
+            for (int u = 0; u < 4; ++u) v[u] = (sv[u] >= 0) ? col[sv[u]] : T(0);
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+                const int t = t0 + u * 256;
+                if (t < nm) vals[c * nm + t] = v[u];
+            }
         }
     }
     __syncthreads();
-    for (int c = 0; c < nc; ++c) {
-        T* col = A + off(0, c0 + c, lda);
-        for (int t = threadIdx.x; t < nm; t += 256) {
-            if (src[t] >= 0) col[dst[t]] = vals[c * nm + t];
+    for (int t0 = threadIdx.x; t0 < nm; t0 += 256 * 4) {
+        int sv[4], dv[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            const int t = t0 + u * 256;
+            sv[u] = (t < nm) ? src[t] : -1;
+            dv[u] = (t < nm) ? dst[t] : 0;
+        }
+        for (int c = 0; c < nc; ++c) {
+            T* col = A + off(0, c0 + c, lda);
+#pragma unroll
+            for (int u = 0; u < 4; ++u)
+                if (sv[u] >= 0) col[dv[u]] = vals[c * nm + t0 + u * 256];
         }
     }
 }

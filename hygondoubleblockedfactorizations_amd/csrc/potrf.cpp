@@ -6,7 +6,10 @@
 // replaced by the gfx950 kernels through the recursive building blocks.
 #include <hip/hip_runtime.h>
 
+#include <type_traits>
+
 #include "blas3.hpp"
+#include "drivers.hpp"
 #include "gemm/update_gemm.h"
 #include "kernels/panel_kernels.h"
 #include "potrf/double_blocked_potrf.h"
@@ -52,9 +55,24 @@ void double_blocked_trsm_impl(mifact_handle h, const T* L, long long ldl, T* B,
     }
 }
 
+// Trailing update C -= A B^T: FP32/FP64 MFMA, or BF16/FP16 MFMA with FP32
+// accumulation for the mixed-precision factorization (FP32 data only).
+template <typename T>
+void update_nt(mifact_handle h, LowPrecision prec, GemmShape shape, int M, int N, int K, const T* A,
+               long long lda, const T* B, long long ldb, T* C, long long ldc)
+{
+    if constexpr (std::is_same_v<T, float>) {
+        gemm_lowp(h, prec, kOpNT, shape, M, N, K, -1.0f, A, lda, B, ldb, C, ldc);
+    } else {
+        gemm<T>(h, kOpNT, shape, M, N, K, T(-1), A, lda, B, ldb, C, ldc);
+    }
+}
+
+}  // namespace
+
 template <typename T>
 void double_blocked_potrf_impl(mifact_handle h, T* A, int lda_i, int n, int nb, int b,
-                               int* info, bool use_dbtrsm)
+                               int* info, bool use_dbtrsm, LowPrecision prec)
 {
     if (h == nullptr) {
         report_status(nullptr, mifact_status_invalid_handle, "double_blocked_potrf: null handle");
@@ -96,9 +114,9 @@ void double_blocked_potrf_impl(mifact_handle h, T* A, int lda_i, int n, int nb, 
             if (panel_start < outer_end) {
                 const int panel_cols = imin(b, outer_end - panel_start);
                 const int k = panel_start - outer;
-                gemm<T>(h, kOpNT, kShapeLower, n - panel_start, panel_cols, k, T(-1),
-                        at(A, lda, panel_start, outer), lda, at(A, lda, panel_start, outer), lda,
-                        at(A, lda, panel_start, panel_start), lda);
+                update_nt<T>(h, prec, kShapeLower, n - panel_start, panel_cols, k,
+                             at(A, lda, panel_start, outer), lda, at(A, lda, panel_start, outer),
+                             lda, at(A, lda, panel_start, panel_start), lda);
             }
         }
         // next outer block -= L * L^T over all previous columns (:251-292):
@@ -107,12 +125,19 @@ void double_blocked_potrf_impl(mifact_handle h, T* A, int lda_i, int n, int nb, 
         const int next_outer = outer_end;
         if (next_outer < n) {
             const int next_size = imin(nb, n - next_outer);
-            gemm<T>(h, kOpNT, kShapeLower, n - next_outer, next_size, next_outer, T(-1),
-                    at(A, lda, next_outer, 0), lda, at(A, lda, next_outer, 0), lda,
-                    at(A, lda, next_outer, next_outer), lda);
+            update_nt<T>(h, prec, kShapeLower, n - next_outer, next_size, next_outer,
+                         at(A, lda, next_outer, 0), lda, at(A, lda, next_outer, 0), lda,
+                         at(A, lda, next_outer, next_outer), lda);
         }
     }
 }
+
+template void double_blocked_potrf_impl<float>(mifact_handle, float*, int, int, int, int, int*,
+                                              bool, LowPrecision);
+template void double_blocked_potrf_impl<double>(mifact_handle, double*, int, int, int, int, int*,
+                                               bool, LowPrecision);
+
+namespace {
 
 template <typename T>
 void syrk_kblocked(mifact_handle h, int n, int k, const T* A, int lda, T* C, int ldc, int k_tile)
@@ -169,26 +194,37 @@ void getrf_panel_impl(mifact_handle h, int m, int n, T* A, int lda, int* ipiv, i
 }  // namespace
 }  // namespace mifact
 
-using mifact::double_blocked_potrf_impl;
+static void potrf_f32(mifact_handle h, float* A, int lda, int n, int nb, int b, int* info,
+                      bool dbtrsm)
+{
+    mifact::double_blocked_potrf_impl<float>(h, A, lda, n, nb, b, info, dbtrsm, mifact::kPrecFP32);
+}
+
+static void potrf_f64(mifact_handle h, double* A, int lda, int n, int nb, int b, int* info,
+                      bool dbtrsm)
+{
+    mifact::double_blocked_potrf_impl<double>(h, A, lda, n, nb, b, info, dbtrsm,
+                                              mifact::kPrecFP32);
+}
 
 extern "C" {
 
 void double_blocked_potrf(mifact_handle handle, float* A, int lda, int n, int nb, int b, int* info)
 {
-    double_blocked_potrf_impl<float>(handle, A, lda, n, nb, b, info, false);
+    potrf_f32(handle, A, lda, n, nb, b, info, false);
 }
 
 void double_blocked_potrf_dbtrsm(mifact_handle handle, float* A, int lda, int n, int nb, int b,
                                  int* info)
 {
-    double_blocked_potrf_impl<float>(handle, A, lda, n, nb, b, info, true);
+    potrf_f32(handle, A, lda, n, nb, b, info, true);
 }
 
 void double_blocked_potrf_kblocked_syrk(mifact_handle handle, float* A, int lda, int n, int nb,
                                         int b, int syrk_k_tile, int* info)
 {
     (void)syrk_k_tile;
-    double_blocked_potrf_impl<float>(handle, A, lda, n, nb, b, info, false);
+    potrf_f32(handle, A, lda, n, nb, b, info, false);
 }
 
 void double_blocked_potrf_kblocked_syrk_adaptive_gemm(mifact_handle handle, float* A, int lda,
@@ -196,7 +232,7 @@ void double_blocked_potrf_kblocked_syrk_adaptive_gemm(mifact_handle handle, floa
                                                       int* info)
 {
     (void)syrk_k_tile;
-    double_blocked_potrf_impl<float>(handle, A, lda, n, nb, b, info, false);
+    potrf_f32(handle, A, lda, n, nb, b, info, false);
 }
 
 void double_blocked_potrf_kblocked_syrk_adaptive_gemm_configured(
@@ -207,26 +243,26 @@ void double_blocked_potrf_kblocked_syrk_adaptive_gemm_configured(
     (void)gemm_m_tile;
     (void)gemm_m_threshold;
     (void)gemm_k_threshold;
-    double_blocked_potrf_impl<float>(handle, A, lda, n, nb, b, info, false);
+    potrf_f32(handle, A, lda, n, nb, b, info, false);
 }
 
 void double_blocked_potrf_double(mifact_handle handle, double* A, int lda, int n, int nb, int b,
                                  int* info)
 {
-    double_blocked_potrf_impl<double>(handle, A, lda, n, nb, b, info, false);
+    potrf_f64(handle, A, lda, n, nb, b, info, false);
 }
 
 void double_blocked_potrf_double_dbtrsm(mifact_handle handle, double* A, int lda, int n, int nb,
                                         int b, int* info)
 {
-    double_blocked_potrf_impl<double>(handle, A, lda, n, nb, b, info, true);
+    potrf_f64(handle, A, lda, n, nb, b, info, true);
 }
 
 void double_blocked_potrf_double_kblocked_syrk(mifact_handle handle, double* A, int lda, int n,
                                                int nb, int b, int syrk_k_tile, int* info)
 {
     (void)syrk_k_tile;
-    double_blocked_potrf_impl<double>(handle, A, lda, n, nb, b, info, false);
+    potrf_f64(handle, A, lda, n, nb, b, info, false);
 }
 
 void double_blocked_potrf_double_kblocked_syrk_adaptive_gemm(mifact_handle handle, double* A,
@@ -234,7 +270,7 @@ void double_blocked_potrf_double_kblocked_syrk_adaptive_gemm(mifact_handle handl
                                                              int syrk_k_tile, int* info)
 {
     (void)syrk_k_tile;
-    double_blocked_potrf_impl<double>(handle, A, lda, n, nb, b, info, false);
+    potrf_f64(handle, A, lda, n, nb, b, info, false);
 }
 
 void double_blocked_potrf_double_kblocked_syrk_adaptive_gemm_configured(
@@ -245,7 +281,7 @@ void double_blocked_potrf_double_kblocked_syrk_adaptive_gemm_configured(
     (void)gemm_m_tile;
     (void)gemm_m_threshold;
     (void)gemm_k_threshold;
-    double_blocked_potrf_impl<double>(handle, A, lda, n, nb, b, info, false);
+    potrf_f64(handle, A, lda, n, nb, b, info, false);
 }
 
 // ------------------------------------------------------------ BLAS-3 blocks

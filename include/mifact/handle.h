@@ -59,6 +59,12 @@ mifact_status mifact_get_last_error(mifact_handle handle);
  * synchronizes the device). */
 mifact_status mifact_reserve_workspace(mifact_handle handle, int n);
 
+/* FP32 update-GEMM tile configuration used by every routine on this handle
+ * (a tuning knob for the autotuner): 0 = automatic (default), 1 = 128x128
+ * tiles with 16-deep K slices, 2 = 128x128 / 32-deep, 3 = 64x64 / 16-deep,
+ * 4 = 256x256 / 16-deep. */
+mifact_status mifact_set_gemm_config(mifact_handle handle, int config);
+
 /* Library version string. */
 const char* mifact_version(void);
 

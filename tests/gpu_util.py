@@ -51,3 +51,26 @@ class Session:
 def gemm_bound(A, B, k):
     """Componentwise FP32 accumulation bound (k + 2) eps |A| |B|."""
     return (k + 2) * EPS32 * (np.abs(A.astype(np.float64)) @ np.abs(B.astype(np.float64)))
+
+
+def pivot_tie_audit(LU, ipiv, ipiv_ref, n, tol):
+    """ipiv parity with the tie rule: identical, or at the FIRST mismatch k
+    the row the reference chose must sit, in our factorization, at a
+    position i > k whose multiplier |L(i, k)| >= 1 - tol, i.e. its candidate
+    magnitude ties ours within the accumulated rounding bound.  After an
+    accepted tie the trajectories legitimately differ (checked by residual).
+    Returns (ok, k or None, multiplier or None)."""
+    mism = np.nonzero(np.asarray(ipiv[:n]) != np.asarray(ipiv_ref[:n]))[0]
+    if len(mism) == 0:
+        return True, None, None
+    k = int(mism[0])
+    q = int(ipiv_ref[k]) - 1            # reference's row position before step k
+    pos = q
+    for s_ in range(k, n):              # follow that row through our swaps
+        p = int(ipiv[s_]) - 1
+        if pos == s_:
+            pos = p
+        elif pos == p:
+            pos = s_
+    mult = abs(float(LU[pos, k])) if pos > k else 0.0
+    return mult >= 1.0 - tol, k, mult

@@ -11,7 +11,7 @@ Bar (written per test):
 import numpy as np
 import pytest
 
-from gpu_util import EPS32, Session, dev, dev_int, host, log, sync
+from gpu_util import EPS32, Session, dev, dev_int, host, log, pivot_tie_audit, sync
 from oracle import oracle as O
 
 pytestmark = pytest.mark.gpu
@@ -209,7 +209,9 @@ def test_getrf_random_vs_oracle(H, n, onb, inb):
     assert info == 0
     X = A.copy(order="F")
     ipiv_ref, info_ref = O.sgetrf(X)
-    assert np.array_equal(ipiv, ipiv_ref)
+    ok, k, mult = pivot_tie_audit(LU, ipiv, ipiv_ref, n, tol=4 * n * EPS32)
+    log("pivot audit", ok, k, mult)
+    assert ok, (k, mult)
     res = O.getrf_residual(LU, A, ipiv)
     log("residual", res, "scaled", res / (n * EPS32))
     assert res <= GETRF_GATE and res / (n * EPS32) <= 10
