@@ -65,8 +65,14 @@ struct mifact_handle_s {
     // host buffer its convergence tests read back.
     void* solve_ws = nullptr;
     size_t solve_ws_bytes = 0;
+    int solve_cap_n = 0;
+    int solve_cap_nrhs = 0;
     void* host_norms = nullptr;
     size_t host_norms_bytes = 0;
+    // Blocked TRSV: epoch of the last launch (block-ready flags carry it)
+    // and the tickets handed out so far (each launch draws nblk of them).
+    int trsv_epoch = 0;
+    unsigned long long trsv_ticket_base = 0;
 
     // Batched path: work-queue order, counter and FP32 staging; the private
     // stream refinement graphs are captured on.

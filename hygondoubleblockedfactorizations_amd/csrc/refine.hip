@@ -48,6 +48,8 @@ struct SolveArena {
     double* Z = nullptr;         // preconditioned directions, ldt x restart
     double* dpart = nullptr;     // dot-product partials
     double* dvals = nullptr;     // [h1 | h2 | ||w||^2 | y] on the device
+    int* flags = nullptr;        // TRSV block-ready flags (epoch tagged)
+    unsigned long long* ticket = nullptr;  // TRSV ticket counter (monotonic)
 };
 
 int residual_chunks(int n)
@@ -88,8 +90,11 @@ size_t arena_bytes(int n, int nrhs, SolveArena* out, char* base)
     a.perm = reinterpret_cast<int*>(take(sizeof(int) * ldt));
     a.pinv = reinterpret_cast<int*>(take(sizeof(int) * ldt));
     a.rowmax = reinterpret_cast<unsigned*>(take(sizeof(unsigned) * ldt));
+    // chunks(m) * round_up(m, 64) <= (kTargetBlocks + tiles) * 64 for every
+    // order m <= n: the partials of any smaller call fit as well
     a.part = reinterpret_cast<double*>(
-        take(sizeof(double) * static_cast<size_t>(residual_chunks(n)) * kRhsPass * ldt));
+        take(sizeof(double) * kRhsPass * 64
+             * (static_cast<size_t>(kTargetBlocks) + static_cast<size_t>(blocks) + 1)));
     a.norms = reinterpret_cast<unsigned long long*>(
         take(sizeof(unsigned long long) * (3 * static_cast<size_t>(nrhs) + 1)));
     a.info = reinterpret_cast<int*>(take(sizeof(int)));
@@ -97,14 +102,22 @@ size_t arena_bytes(int n, int nrhs, SolveArena* out, char* base)
     a.Z = reinterpret_cast<double*>(take(sizeof(double) * ldt * kGmresRestart));
     a.dpart = reinterpret_cast<double*>(take(sizeof(double) * kDotChunks * (kGmresRestart + 1)));
     a.dvals = reinterpret_cast<double*>(take(sizeof(double) * (4 * kGmresRestart + 8)));
+    a.flags = reinterpret_cast<int*>(take(sizeof(int) * blocks));
+    a.ticket = reinterpret_cast<unsigned long long*>(take(sizeof(unsigned long long)));
     if (out != nullptr) *out = a;
     return o;
 }
 
+// The arena's layout is a function of its capacity (largest order and
+// right-hand-side count requested so far), so every offset -- in particular
+// the TRSV flags and ticket counter -- stays put between calls until the
+// arena is reallocated (and zeroed) for a larger capacity.
 bool ensure_arena(mifact_handle h, int n, int nrhs, SolveArena* a)
 {
-    const size_t need = arena_bytes(n, nrhs, nullptr, nullptr);
-    if (h->solve_ws == nullptr || h->solve_ws_bytes < need) {
+    const int cap_n = imax(h->solve_cap_n, imax(n, 1));
+    const int cap_r = imax(h->solve_cap_nrhs, imax(nrhs, 1));
+    const size_t need = arena_bytes(cap_n, cap_r, nullptr, nullptr);
+    if (h->solve_ws == nullptr || cap_n != h->solve_cap_n || cap_r != h->solve_cap_nrhs) {
         if (h->solve_ws != nullptr) {
             if (hipStreamSynchronize(h->stream) != hipSuccess) {
                 report_status(h, mifact_status_hip_error, "stream sync before solver arena growth");
@@ -119,10 +132,20 @@ bool ensure_arena(mifact_handle h, int n, int nrhs, SolveArena* a)
             report_status(h, mifact_status_allocation_failed, "solver arena allocation failed");
             return false;
         }
+        // zero once: TRSV flags (epochs start at 1) and the ticket counter
+        if (hipMemset(p, 0, need) != hipSuccess) {
+            (void)hipFree(p);
+            report_status(h, mifact_status_hip_error, "solver arena initialization failed");
+            return false;
+        }
         h->solve_ws = p;
         h->solve_ws_bytes = need;
+        h->solve_cap_n = cap_n;
+        h->solve_cap_nrhs = cap_r;
+        h->trsv_epoch = 0;
+        h->trsv_ticket_base = 0;
     }
-    const size_t host_need = host_gmres_offset(nrhs) + sizeof(double) * (4 * kGmresRestart + 8);
+    const size_t host_need = host_gmres_offset(cap_r) + sizeof(double) * (4 * kGmresRestart + 8);
     if (h->host_norms == nullptr || h->host_norms_bytes < host_need) {
         if (h->host_norms != nullptr) {
             (void)hipHostFree(h->host_norms);
@@ -136,7 +159,7 @@ bool ensure_arena(mifact_handle h, int n, int nrhs, SolveArena* a)
         h->host_norms = p;
         h->host_norms_bytes = host_need;
     }
-    arena_bytes(n, nrhs, a, static_cast<char*>(h->solve_ws));
+    arena_bytes(cap_n, cap_r, a, static_cast<char*>(h->solve_ws));
     return true;
 }
 
@@ -519,6 +542,135 @@ __global__ void absmax_kernel(const double* x, int n, unsigned long long* out)
     if (i < n) atomicMax(out, abs_bits(x[i]));
 }
 
+// ---------------------------------------------------------------- blocked TRSV
+
+// Triangular solves with few right-hand sides in one launch.  Row block i
+// (64 rows) is owned by the workgroup that draws ticket t (t = i forward,
+// nblk - 1 - i backward); it accumulates M_ij x_j over its dependency blocks
+// as their ready flags appear, then applies the 64 x 64 diagonal inverse and
+// publishes x_i with an epoch-tagged flag.  Tickets are handed out in
+// dispatch order, so every block waited on belongs to a workgroup that is
+// already running: the chain always progresses.  Waits are bounded; on
+// expiry the kernel raises the handle's status word and finishes.
+enum TrsvMode { kFwdLower = 0, kBwdUpper = 1, kBwdLowerT = 2 };
+
+constexpr int kTrsvSpin = 1 << 24;
+
+template <int MODE, int NR>
+__global__ __launch_bounds__(256) void trsv_kernel(const float* F, long long ldf, int n,
+                                                   const float* inv, float* T, long long ldt,
+                                                   int nr, int* flags, int epoch,
+                                                   unsigned long long* ticket,
+                                                   unsigned long long ticket_base, int* status)
+{
+    __shared__ float tile[64][65];
+    __shared__ float xs[64][NR];
+    __shared__ float part[4][64][NR];
+    __shared__ int bsh;
+    const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+    const int nblk = (n + 63) / 64;
+    if (tid == 0) {
+        const unsigned long long t = atomicAdd(ticket, 1ull) - ticket_base;
+        bsh = MODE == kFwdLower ? static_cast<int>(t) : nblk - 1 - static_cast<int>(t);
+    }
+    __syncthreads();
+    const int bi = bsh;
+    if (bi < 0 || bi >= nblk) return;
+    const int r0 = bi * 64;
+    const int rb = imin(64, n - r0);
+    float acc[NR];
+#pragma unroll
+    for (int k = 0; k < NR; ++k) acc[k] = 0.0f;
+
+    const int ndep = MODE == kFwdLower ? bi : nblk - 1 - bi;
+    for (int q = 0; q < ndep; ++q) {
+        const int bj = MODE == kFwdLower ? q : nblk - 1 - q;
+        const int c0 = bj * 64;
+        const int cb = imin(64, n - c0);
+        // source block of M_ij (Lower-T reads L_ji and transposes)
+        const int sr = MODE == kBwdLowerT ? c0 : r0;
+        const int sc = MODE == kBwdLowerT ? r0 : c0;
+        const int srows = MODE == kBwdLowerT ? cb : rb;
+        const int scols = MODE == kBwdLowerT ? rb : cb;
+#pragma unroll
+        for (int e = 0; e < 16; ++e) {
+            const int idx = tid + 256 * e;
+            const int lr = idx & 63, lc = idx >> 6;
+            tile[lc][lr] = (lr < srows && lc < scols) ? F[off(sr + lr, sc + lc, ldf)] : 0.0f;
+        }
+        // wait for x_j
+        if (tid == 0) {
+            int spins = 0;
+            while (__hip_atomic_load(flags + bj, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT)
+                   != epoch) {
+                if (++spins > kTrsvSpin) {
+                    atomicExch(status, 3);
+                    break;
+                }
+                __builtin_amdgcn_s_sleep(1);
+            }
+        }
+        __syncthreads();
+        for (int e = tid; e < 64 * NR; e += 256) {
+            const int c = e / NR, k = e % NR;
+            xs[c][k] = (c < cb && k < nr) ? T[off(c0 + c, k, ldt)] : 0.0f;
+        }
+        __syncthreads();
+#pragma unroll 4
+        for (int cc = 0; cc < 16; ++cc) {
+            const int c = w * 16 + cc;
+            const float m = MODE == kBwdLowerT ? tile[lane][c] : tile[c][lane];
+#pragma unroll
+            for (int k = 0; k < NR; ++k) acc[k] = fmaf(m, xs[c][k], acc[k]);
+        }
+        __syncthreads();
+    }
+    // s = b_i - sum; x_i = D_i^{-1} s with the stored inverse block
+#pragma unroll
+    for (int k = 0; k < NR; ++k) part[w][lane][k] = acc[k];
+#pragma unroll
+    for (int e = 0; e < 16; ++e) {
+        const int idx = tid + 256 * e;
+        const int lr = idx & 63, lc = idx >> 6;
+        tile[lc][lr] = inv[static_cast<size_t>(bi) * 4096 + lc * 64 + lr];
+    }
+    __syncthreads();
+    for (int e = tid; e < 64 * NR; e += 256) {
+        const int r = e / NR, k = e % NR;
+        float v = 0.0f;
+        if (r < rb && k < nr) {
+            v = T[off(r0 + r, k, ldt)]
+              - ((part[0][r][k] + part[1][r][k]) + (part[2][r][k] + part[3][r][k]));
+        }
+        xs[r][k] = v;
+    }
+    __syncthreads();
+    // x(r) = sum_c Dinv(r, c) s(c); backward modes hold the transposed inverse
+    float y[NR];
+#pragma unroll
+    for (int k = 0; k < NR; ++k) y[k] = 0.0f;
+#pragma unroll 4
+    for (int cc = 0; cc < 16; ++cc) {
+        const int c = w * 16 + cc;
+        const float d = (MODE == kFwdLower) ? tile[c][lane] : tile[lane][c];
+#pragma unroll
+        for (int k = 0; k < NR; ++k) y[k] = fmaf(d, xs[c][k], y[k]);
+    }
+#pragma unroll
+    for (int k = 0; k < NR; ++k) part[w][lane][k] = y[k];
+    __syncthreads();
+    for (int e = tid; e < 64 * NR; e += 256) {
+        const int r = e / NR, k = e % NR;
+        if (r < rb && k < nr) {
+            T[off(r0 + r, k, ldt)] =
+                (part[0][r][k] + part[1][r][k]) + (part[2][r][k] + part[3][r][k]);
+        }
+    }
+    __threadfence();
+    __syncthreads();
+    if (tid == 0) __hip_atomic_store(flags + bi, epoch, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+}
+
 // ---------------------------------------------------------------- host pieces
 
 void ipiv_to_perm(mifact_handle h, const int* ipiv, int n, int* perm)
@@ -651,19 +803,52 @@ void equilibrate(mifact_handle h, bool sym, const float* A, long long lda, int n
     MIFACT_LAUNCHED(h);
 }
 
-// FP32 solves with the factors and the inverses of their diagonal blocks.
-void solve_chol(mifact_handle h, int n, int nrhs, const float* L, long long ldl, float* B,
-                long long ldb, const float* inv)
+// One blocked TRSV launch per pass of <= 16 right-hand sides.
+void trsv(mifact_handle h, TrsvMode mode, int n, int nrhs, const float* F, long long ldf,
+          float* T, long long ldt, const float* inv, const SolveArena& w)
 {
-    trsm_llu_inv<float>(h, n, nrhs, L, ldl, B, ldb, inv);
-    trsm_llt_inv<float>(h, n, nrhs, L, ldl, B, ldb, inv);
+    const int nblk = ceil_div(n, 64);
+    for (int k0 = 0; k0 < nrhs; k0 += kRhsPass) {
+        const int nr = imin(kRhsPass, nrhs - k0);
+        h->trsv_epoch += 1;
+        if (h->trsv_epoch <= 0) h->trsv_epoch = 1;
+        const int epoch = h->trsv_epoch;
+        const unsigned long long base = h->trsv_ticket_base;
+        h->trsv_ticket_base += static_cast<unsigned long long>(nblk);
+        float* Tk = T + off(0, k0, ldt);
+        auto go = [&](auto mode_tag, auto nr_tag) {
+            constexpr int M = decltype(mode_tag)::value;
+            constexpr int R = decltype(nr_tag)::value;
+            hipLaunchKernelGGL((trsv_kernel<M, R>), dim3(nblk), dim3(256), 0, h->stream, F, ldf,
+                               n, inv, Tk, ldt, nr, w.flags, epoch, w.ticket, base,
+                               h->kernel_status);
+        };
+        auto by_nr = [&](auto mode_tag) {
+            if (nr == 1) go(mode_tag, std::integral_constant<int, 1>{});
+            else if (nr <= 4) go(mode_tag, std::integral_constant<int, 4>{});
+            else go(mode_tag, std::integral_constant<int, kRhsPass>{});
+        };
+        if (mode == kFwdLower) by_nr(std::integral_constant<int, kFwdLower>{});
+        else if (mode == kBwdUpper) by_nr(std::integral_constant<int, kBwdUpper>{});
+        else by_nr(std::integral_constant<int, kBwdLowerT>{});
+        MIFACT_LAUNCHED(h);
+    }
+}
+
+// FP32 solves with the factors and the inverses of their diagonal blocks:
+// blocked single-launch TRSV per triangle.
+void solve_chol(mifact_handle h, int n, int nrhs, const float* L, long long ldl, float* B,
+                long long ldb, const float* inv, const SolveArena& w)
+{
+    trsv(h, kFwdLower, n, nrhs, L, ldl, B, ldb, inv, w);
+    trsv(h, kBwdLowerT, n, nrhs, L, ldl, B, ldb, inv, w);
 }
 
 void solve_lu(mifact_handle h, int n, int nrhs, const float* LU, long long ldlu, float* B,
-              long long ldb, const float* inv_l, const float* inv_ut)
+              long long ldb, const float* inv_l, const float* inv_ut, const SolveArena& w)
 {
-    trsm_llu_inv<float>(h, n, nrhs, LU, ldlu, B, ldb, inv_l);
-    trsm_lun_inv<float>(h, n, nrhs, LU, ldlu, B, ldb, inv_ut);
+    trsv(h, kFwdLower, n, nrhs, LU, ldlu, B, ldb, inv_l, w);
+    trsv(h, kBwdUpper, n, nrhs, LU, ldlu, B, ldb, inv_ut, w);
 }
 
 double bits_to_double(unsigned long long b)
@@ -723,8 +908,8 @@ void apply_precond(mifact_handle h, const Precond& P, const double* v, double* z
     hipLaunchKernelGGL(to_fp32_kernel, g, b, 0, h->stream, v, n, P.rs, P.pinv, P.w->t);
     {
         StageScope scope(h, kStageSolve, 2.0 * n * n, 4.0 * n * n);
-        if (P.sym) solve_chol(h, n, 1, P.F, P.ldf, P.w->t, P.w->ldt, P.w->inv_a);
-        else solve_lu(h, n, 1, P.F, P.ldf, P.w->t, P.w->ldt, P.w->inv_a, P.w->inv_b);
+        if (P.sym) solve_chol(h, n, 1, P.F, P.ldf, P.w->t, P.w->ldt, P.w->inv_a, *P.w);
+        else solve_lu(h, n, 1, P.F, P.ldf, P.w->t, P.w->ldt, P.w->inv_a, P.w->inv_b, *P.w);
     }
     hipLaunchKernelGGL(from_fp32_kernel, g, b, 0, h->stream, P.w->t, n, P.cs, z);
 }
@@ -968,8 +1153,8 @@ mifact_status refine_solve(mifact_handle h, bool sym, int n, int nrhs, const flo
         for (int it = 0;; ++it) {
             {
                 StageScope scope(h, kStageSolve, 2.0 * n * n * nrhs, 4.0 * n * n);
-                if (sym) solve_chol(h, n, nrhs, F, ldf, w.t, w.ldt, w.inv_a);
-                else solve_lu(h, n, nrhs, F, ldf, w.t, w.ldt, w.inv_a, w.inv_b);
+                if (sym) solve_chol(h, n, nrhs, F, ldf, w.t, w.ldt, w.inv_a, w);
+                else solve_lu(h, n, nrhs, F, ldf, w.t, w.ldt, w.inv_a, w.inv_b, w);
             }
             fill<unsigned long long>(h, w.norms, 2 * nrhs, 2 * nrhs, 1, 0ull);
             hipLaunchKernelGGL(update_kernel, dim3(ceil_div(n, 256), nrhs), dim3(256), 0,
@@ -1097,7 +1282,7 @@ void mifact_potrs(mifact_handle handle, int n, int nrhs, const float* L, int ldl
     if (!ensure_arena(handle, n, 1, &w)) return;
     trtri_blocks<float>(handle, n, L, ldl, w.inv_a, false, false);
     StageScope scope(handle, kStageSolve, 2.0 * n * n * nrhs, 4.0 * n * n);
-    solve_chol(handle, n, nrhs, L, ldl, B, ldb, w.inv_a);
+    solve_chol(handle, n, nrhs, L, ldl, B, ldb, w.inv_a, w);
 }
 
 void mifact_getrs(mifact_handle handle, int n, int nrhs, const float* LU, int ldlu,
@@ -1116,7 +1301,7 @@ void mifact_getrs(mifact_handle handle, int n, int nrhs, const float* LU, int ld
     gather_rows(handle, B, ldb, n, nrhs, w.perm, w.t, w.ldt);
     {
         StageScope scope(handle, kStageSolve, 2.0 * n * n * nrhs, 4.0 * n * n);
-        solve_lu(handle, n, nrhs, LU, ldlu, w.t, w.ldt, w.inv_a, w.inv_b);
+        solve_lu(handle, n, nrhs, LU, ldlu, w.t, w.ldt, w.inv_a, w.inv_b, w);
     }
     gather_rows(handle, w.t, w.ldt, n, nrhs, nullptr, B, ldb);
 }
