@@ -29,6 +29,12 @@ import time
 
 METRIC = "POTRF/GETRF TFLOP/s @n=32768; mixed+IR vs FP32 solve time, same accuracy"
 FP32_PEAK_TFLOPS = 157.3      # MI355X dense FP32 (MFMA)
+# HBM bytes per dispatch of the FP32 lower-trapezoid update GEMM at n=32768,
+# nb/b=8192/1024: rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE runs of this script
+# (POTRF only), scaled by the same counters' reading of the bench's own
+# n^2 x 4 B input copy (factors 2.0 and 1.0); see
+# profiles/gemm_kernel_potrf_n32768_pmc.txt.
+TRAPEZOID_GEMM_TRAFFIC = {(32768, 8192, 1024): 6.3909e9}
 HBM_PEAK_GBS = 8000.0
 
 
@@ -458,9 +464,13 @@ def main():
         roofline = {"bound": "mfma", "achieved": achieved, "peak": FP32_PEAK_TFLOPS,
                     "unit": "TFLOP/s", "frac": achieved / FP32_PEAK_TFLOPS, "traffic": None,
                     "kernel": "mifact::gemm_kernel<float> (MFMA 32x32x2 f32)",
+                    "traffic_kernel": "gemm_kernel<float,128,128,16,NT,lower> (POTRF trapezoid "
+                                      "update), HBM bytes per dispatch",
                     "launches": g["calls"],
                     "avg_launch_ms": g["ms"] / max(g["calls"], 1),
                     "flops_per_launch": g["flops"] / max(g["calls"], 1)}
+    if roofline is not None:
+        roofline["traffic"] = TRAPEZOID_GEMM_TRAFFIC.get((n, pnb, pb))
     stage_out = {k: {"calls": v["calls"], "ms": v["ms"] / max(args.steps, 1),
                      "tflops": (v["flops"] / (v["ms"] * 1e-3) / 1e12) if v["ms"] > 0 else None,
                      "gbs": (v["bytes"] / (v["ms"] * 1e-3) / 1e9) if v["ms"] > 0 else None}

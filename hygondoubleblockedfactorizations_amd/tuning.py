@@ -1,0 +1,148 @@
+"""Autotuner for the double-blocked drivers' block sizes.
+
+The reference picks nb/b by hand per run (its benchmark scripts pass them on
+the command line; potrf_double_blocked_cases.cpp:12-114,
+getrf_double_blocked_cases.cpp:13-109).  Here the outer/inner block sizes of
+POTRF and GETRF and the GETRF outer-TRSM block are measured on the device:
+every candidate of a small grid is timed (HIP events, median of `reps` runs
+after one warm-up) on the same seeded inputs as the benchmark, and the
+fastest is stored in ``tuning_table.json`` next to this file, keyed by n.
+``lookup(n)`` returns the entry of the nearest tuned order (block sizes
+clipped to n), or the built-in defaults when no table exists.
+
+    python -m hygondoubleblockedfactorizations_amd.tuning --n 8192 16384 32768
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+TABLE = os.path.join(HERE, "tuning_table.json")
+
+# Candidate grids (outer, inner) per routine; the outer-TRSM block of GETRF.
+POTRF_GRID = [(2048, 256), (4096, 256), (4096, 512), (8192, 256), (8192, 512), (8192, 1024),
+              (12288, 512), (16384, 512), (16384, 1024)]
+GETRF_GRID = [(4096, 2048), (4096, 4096), (6144, 3072), (8192, 2048), (8192, 4096),
+              (16384, 4096)]
+TRSM_BLOCKS = [2048, 4096]
+
+
+def defaults(n: int) -> dict:
+    """Block sizes used when nothing is tuned for n."""
+    return {
+        "potrf_nb": min(8192, n), "potrf_b": min(1024, n),
+        "getrf_nb": min(max(n // 2, 1), 20480), "getrf_b": min(max(n // 4, 1), 4096),
+        "trsm_block": 2048,
+    }
+
+
+def load_table(path: str = TABLE) -> dict:
+    if not os.path.exists(path):
+        return {}
+    with open(path) as f:
+        return {int(k): v for k, v in json.load(f).items()}
+
+
+def lookup(n: int, path: str = TABLE) -> dict:
+    """Tuned block sizes for order n (nearest tuned order, clipped to n)."""
+    table = load_table(path)
+    if not table:
+        return defaults(n)
+    key = min(table, key=lambda m: abs(m - n))
+    e = dict(table[key])
+    for k in ("potrf_nb", "potrf_b", "getrf_nb", "getrf_b"):
+        e[k] = max(1, min(int(e[k]), n))
+    e["potrf_b"] = min(e["potrf_b"], e["potrf_nb"])
+    e["getrf_b"] = min(e["getrf_b"], e["getrf_nb"])
+    return {k: e[k] for k in ("potrf_nb", "potrf_b", "getrf_nb", "getrf_b", "trsm_block")}
+
+
+def _time(torch, fn, reset, reps):
+    ev0 = torch.cuda.Event(enable_timing=True)
+    ev1 = torch.cuda.Event(enable_timing=True)
+    out = []
+    for i in range(reps + 1):
+        reset()
+        torch.cuda.synchronize()
+        ev0.record()
+        fn()
+        ev1.record()
+        torch.cuda.synchronize()
+        if i > 0:
+            out.append(ev0.elapsed_time(ev1))
+    out.sort()
+    return out[len(out) // 2]
+
+
+def tune(n: int, reps: int = 2, log=print) -> dict:
+    """Times the candidate grids at order n on the current GPU."""
+    import torch
+    from . import _lib
+    lib = _lib.load()
+    h = _lib.Handle(stream=torch.cuda.current_stream())
+    h.reserve(n)
+    P = _lib.ptr
+    g = torch.Generator(device="cuda")
+    g.manual_seed(20260716)
+    G = torch.randn(n, n, device="cuda", generator=g)
+    spd = torch.addmm(torch.eye(n, device="cuda"), G, G.t(), alpha=1.0 / n)
+    del G
+    gen = torch.rand(n, n, device="cuda", generator=g) * 2 - 1
+    work = torch.empty_like(spd)
+    info = torch.zeros(1, dtype=torch.int32, device="cuda")
+    ipiv = torch.zeros(n, dtype=torch.int32, device="cuda")
+    best = {}
+    res = []
+    for nb, b in POTRF_GRID:
+        if nb > n or b > nb:
+            continue
+        ms = _time(torch, lambda: lib.double_blocked_potrf(h.ptr, P(work), n, n, nb, b, P(info)),
+                   lambda: work.copy_(spd), reps)
+        res.append((ms, nb, b))
+        log(f"  potrf n={n} nb={nb} b={b}: {ms:.2f} ms")
+    ms, nb, b = min(res)
+    best.update(potrf_nb=nb, potrf_b=b, potrf_ms=ms)
+    res = []
+    for nb, b in GETRF_GRID:
+        if nb > n or b > nb:
+            continue
+        for tb in TRSM_BLOCKS:
+            if tb > nb:
+                continue
+            ms = _time(torch, lambda: lib.double_blocked_getrf(h.ptr, P(work), n, n, nb, b, 1, tb,
+                                                               P(ipiv), P(info)),
+                       lambda: work.copy_(gen), reps)
+            res.append((ms, nb, b, tb))
+            log(f"  getrf n={n} nb={nb} b={b} tb={tb}: {ms:.2f} ms")
+    if res:
+        ms, nb, b, tb = min(res)
+        best.update(getrf_nb=nb, getrf_b=b, trsm_block=tb, getrf_ms=ms)
+    else:
+        d = defaults(n)
+        best.update(getrf_nb=d["getrf_nb"], getrf_b=d["getrf_b"], trsm_block=d["trsm_block"])
+    if h.last_error() != 0:
+        raise RuntimeError("library reported an error while tuning")
+    return best
+
+
+def main(argv=None):
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--n", type=int, nargs="+", default=[8192, 16384, 32768])
+    ap.add_argument("--reps", type=int, default=2)
+    ap.add_argument("--out", default=TABLE)
+    args = ap.parse_args(argv)
+    table = load_table(args.out)
+    for n in args.n:
+        print(f"tuning n={n}", flush=True)
+        table[n] = tune(n, args.reps, log=lambda s: print(s, flush=True))
+        print(f"  best: {table[n]}", flush=True)
+        with open(args.out, "w") as f:
+            json.dump({str(k): v for k, v in sorted(table.items())}, f, indent=1)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())
