@@ -9,9 +9,11 @@ every factorization; each factorization is bracketed by synchronize + HIP
 events on the library's stream.  value = (n^3/3 + 2 n^3/3) / summed time.
 
 Also reported:
-  * roofline of the dominant kernel (the MFMA update GEMM), from the
-    library's stage profiler (HIP event pairs around every launch, on the
-    launch stream, over the timed steps);
+  * roofline of the dominant kernel (the large-tile MFMA trailing-update
+    GEMM), from the library's stage profiler: HIP event pairs on the launch
+    stream around each of its launches in the timed steps (only that stage is
+    recorded there; a further untimed step records every stage for the
+    breakdown);
   * cpu_baseline: LAPACK spotrf + sgetrf (the oracle's LAPACK loader,
     OpenBLAS) on host cores, on a bounded sample of the same input families;
   * the FP64 residual ||A - L L^T||_F/||A||_F and ||P A - L U||_F/||A||_F of
@@ -29,12 +31,12 @@ import time
 
 METRIC = "POTRF/GETRF TFLOP/s @n=32768; mixed+IR vs FP32 solve time, same accuracy"
 FP32_PEAK_TFLOPS = 157.3      # MI355X dense FP32 (MFMA)
-# HBM bytes per dispatch of the FP32 lower-trapezoid update GEMM at n=32768,
-# nb/b=8192/1024: rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE runs of this script
+# HBM bytes per dispatch of the FP32 lower-trapezoid update GEMM of POTRF at
+# (n, nb, b): rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE runs of this script
 # (POTRF only), scaled by the same counters' reading of the bench's own
 # n^2 x 4 B input copy (factors 2.0 and 1.0); see
-# profiles/gemm_kernel_potrf_n32768_pmc.txt.
-TRAPEZOID_GEMM_TRAFFIC = {(32768, 8192, 1024): 6.3909e9}
+# profiles/gemm_update_potrf_n32768_nb8192_b256_pmc.txt.
+TRAPEZOID_GEMM_TRAFFIC = {(32768, 8192, 256): 4.2212e10}
 HBM_PEAK_GBS = 8000.0
 
 
@@ -63,6 +65,9 @@ def parse():
     ap.add_argument("--no-solve", action="store_true",
                     help="skip the mixed+IR vs FP32 solve comparison")
     ap.add_argument("--solve-precision", default="fp16", choices=["bf16", "fp16"])
+    ap.add_argument("--mixed-b", type=int, default=1024,
+                    help="inner block of the mixed-precision factorization (its BF16/FP16 "
+                         "updates favour wider panels than the FP32 path)")
     ap.add_argument("--solve-kappa", type=float, default=1e4,
                     help="condition number of the SPD solve matrix Q diag(sigma) Q^T")
     ap.add_argument("--no-batched", action="store_true", help="skip the batched section")
@@ -176,7 +181,7 @@ def cpu_baseline(torch, args):
     }
 
 
-def solve_compare(torch, lib, h, P, n, routine, A, nb, b, tb, prec, seed):
+def solve_compare(torch, lib, h, P, n, routine, A, nb, b, tb, prec, seed, mixed_b):
     """Time-to-solution of the plain FP32 solve (FP32 factorization + FP32
     triangular solves) against the mixed-precision factorization with FP64
     refinement run to the FP32 solve's own backward error."""
@@ -227,8 +232,8 @@ def solve_compare(torch, lib, h, P, n, routine, A, nb, b, tb, prec, seed):
     log(f"{routine} FP32 solve {t32:.1f} ms berr {berr32:.2e}")
     # mixed + FP64 refinement to the same backward error
     x = torch.zeros_like(b64)
-    prm = _lib.RefineParams(_lib.PRECISION[prec], nb, b, 1, tb, 30, berr32, 1,
-                            _lib.REFINE_METHOD["auto"])
+    prm = _lib.RefineParams(_lib.PRECISION[prec], nb, max(b, min(mixed_b, nb)), 1, tb, 30, berr32,
+                            1, _lib.REFINE_METHOD["auto"])
     res = _lib.RefineResult()
     for rep in range(2):
         torch.cuda.synchronize()
@@ -409,8 +414,11 @@ def main():
             run(r)
             torch.cuda.synchronize()
             log(f"warmup {r} {i} done")
+    # Timed steps: the stage profiler brackets only the dominant kernel (the
+    # large-tile trailing-update GEMM, a few hundred launches per step), so
+    # its event pairs do not perturb the step time.
     if not args.no_profile:
-        _lib.profile_begin(h)
+        _lib.profile_begin(h, ["gemm_update"])
     for i in range(args.steps):
         for r in routines:
             work.copy_(inputs[r])
@@ -421,7 +429,16 @@ def main():
             torch.cuda.synchronize()
             times[r].append(ev0.elapsed_time(ev1))
             log(f"step {i} {r}: {times[r][-1]:.2f} ms")
+    dominant = {}
     if not args.no_profile:
+        dominant = _lib.profile_end(h)
+        # one further (untimed) step with every stage recorded: the breakdown
+        _lib.profile_begin(h)
+        for r in routines:
+            work.copy_(inputs[r])
+            torch.cuda.synchronize()
+            run(r)
+            torch.cuda.synchronize()
         stages = _lib.profile_end(h)
     st = h.last_error()
     if st != 0:
@@ -458,20 +475,21 @@ def main():
     value = total_flops / (total_ms * 1e-3) / 1e12
 
     roofline = None
-    g = stages.get("gemm")
+    g = dominant.get("gemm_update")
     if g and g["ms"] > 0:
         achieved = g["flops"] / (g["ms"] * 1e-3) / 1e12
         roofline = {"bound": "mfma", "achieved": achieved, "peak": FP32_PEAK_TFLOPS,
                     "unit": "TFLOP/s", "frac": achieved / FP32_PEAK_TFLOPS, "traffic": None,
-                    "kernel": "mifact::gemm_kernel<float> (MFMA 32x32x2 f32)",
-                    "traffic_kernel": "gemm_kernel<float,128,128,16,NT,lower> (POTRF trapezoid "
-                                      "update), HBM bytes per dispatch",
+                    "kernel": "mifact::gemm_kernel<float,128,128,16,...> trailing updates "
+                              "(MFMA 32x32x2 f32), all launches of the timed steps",
+                    "traffic_kernel": "gemm_kernel<float,128,128,16,NT,lower>: POTRF outer "
+                                      "trapezoid updates, HBM bytes per dispatch",
                     "launches": g["calls"],
                     "avg_launch_ms": g["ms"] / max(g["calls"], 1),
                     "flops_per_launch": g["flops"] / max(g["calls"], 1)}
     if roofline is not None:
         roofline["traffic"] = TRAPEZOID_GEMM_TRAFFIC.get((n, pnb, pb))
-    stage_out = {k: {"calls": v["calls"], "ms": v["ms"] / max(args.steps, 1),
+    stage_out = {k: {"calls": v["calls"], "ms": v["ms"],
                      "tflops": (v["flops"] / (v["ms"] * 1e-3) / 1e12) if v["ms"] > 0 else None,
                      "gbs": (v["bytes"] / (v["ms"] * 1e-3) / 1e9) if v["ms"] > 0 else None}
                  for k, v in stages.items()}
@@ -486,12 +504,12 @@ def main():
                     A = spd_kappa(torch, n, args.solve_kappa, args.seed + 7)
                     log(f"kappa={args.solve_kappa:g} SPD matrix in {time.perf_counter() - t_gen:.1f} s")
                     solve[r] = solve_compare(torch, lib, h, P, n, r, A, pnb, pb, tb,
-                                             args.solve_precision, args.seed + 8)
+                                             args.solve_precision, args.seed + 8, args.mixed_b)
                     solve[r]["matrix"] = f"Q diag(sigma) Q^T, kappa={args.solve_kappa:g}"
                     del A
                 else:
                     solve[r] = solve_compare(torch, lib, h, P, n, r, inputs[r], gnb, gb, tb,
-                                             args.solve_precision, args.seed + 9)
+                                             args.solve_precision, args.seed + 9, args.mixed_b)
                     solve[r]["matrix"] = "iid Uniform(-1,1)"
         except Exception as e:  # report, never hide the GPU line
             log(f"solve comparison failed: {e}")

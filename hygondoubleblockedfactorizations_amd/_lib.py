@@ -28,6 +28,7 @@ SIGNATURES = {
     "mifact_set_gemm_config": [P, I],
     "mifact_version": [],
     "mifact_profile_begin": [P],
+    "mifact_profile_begin_stages": [P, ctypes.c_uint],
     "mifact_profile_end": [P, P, I, ctypes.POINTER(I)],
     # potrf/double_blocked_potrf.h
     "double_blocked_potrf": [P, P, I, I, I, I, P],
@@ -104,6 +105,7 @@ _RESTYPE = {
     "mifact_set_gemm_config": I,
     "mifact_version": ctypes.c_char_p,
     "mifact_profile_begin": I,
+    "mifact_profile_begin_stages": I,
     "mifact_profile_end": I,
     "mifact_posv_refine": I,
     "mifact_gesv_refine": I,
@@ -198,8 +200,19 @@ class StageStats(ctypes.Structure):
     ]
 
 
-def profile_begin(handle):
-    load().mifact_profile_begin(handle.ptr)
+STAGES = ["gemm", "potf2", "trsm_leaf", "getf2_leaf", "row_moves", "perm", "gemm_lowp",
+          "solve", "batched", "gemm_update"]
+
+
+def profile_begin(handle, stages=None):
+    """Starts the stage profiler (all stages, or only the named ones)."""
+    if stages is None:
+        load().mifact_profile_begin(handle.ptr)
+    else:
+        mask = 0
+        for s_ in stages:
+            mask |= 1 << STAGES.index(s_)
+        load().mifact_profile_begin_stages(handle.ptr, mask)
 
 
 def profile_end(handle):

@@ -24,6 +24,7 @@ enum Stage {
     kStageGemmLowp,
     kStageSolve,
     kStageBatched,
+    kStageGemmUpdate,
     kStageCount
 };
 
@@ -82,6 +83,7 @@ struct mifact_handle_s {
 
     // Stage profiler: event pool and the records of the current session.
     bool profiling = false;
+    unsigned profile_mask = ~0u;   // stages recorded while profiling (bit per Stage)
     std::vector<hipEvent_t> event_pool;
     size_t events_used = 0;
     std::vector<mifact::ProfileRecord> records;

@@ -112,7 +112,7 @@ static hipEvent_t next_event(mifact_handle h)
 
 StageScope::StageScope(mifact_handle h, Stage stage, double flops, double bytes) : h_(h)
 {
-    if (h == nullptr || !h->profiling) return;
+    if (h == nullptr || !h->profiling || ((h->profile_mask >> stage) & 1u) == 0) return;
     hipEvent_t a = next_event(h);
     hipEvent_t b = next_event(h);
     if (a == nullptr || b == nullptr) return;
@@ -133,10 +133,16 @@ extern "C" {
 
 mifact_status mifact_profile_begin(mifact_handle h)
 {
+    return mifact_profile_begin_stages(h, ~0u);
+}
+
+mifact_status mifact_profile_begin_stages(mifact_handle h, unsigned stage_mask)
+{
     if (h == nullptr) return mifact_status_invalid_handle;
     h->records.clear();
     h->events_used = 0;
     h->profiling = true;
+    h->profile_mask = stage_mask;
     return mifact_status_success;
 }
 
@@ -147,7 +153,7 @@ mifact_status mifact_profile_end(mifact_handle h, mifact_stage_stats* stats, int
     h->profiling = false;
     static const char* names[mifact::kStageCount] = {"gemm", "potf2", "trsm_leaf", "getf2_leaf",
                                                      "row_moves", "perm", "gemm_lowp", "solve",
-                                                     "batched"};
+                                                     "batched", "gemm_update"};
     mifact_stage_stats acc[mifact::kStageCount];
     for (int s = 0; s < mifact::kStageCount; ++s) {
         std::snprintf(acc[s].name, sizeof(acc[s].name), "%s", names[s]);

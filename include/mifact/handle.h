@@ -76,7 +76,9 @@ const char* mifact_version(void);
  * stream and tagged with its stage and algorithmic work; mifact_profile_end
  * synchronizes once and resolves the times. */
 typedef struct mifact_stage_stats {
-    char name[32];   /* stage: gemm, potf2, trsm_leaf, getf2_leaf, row_moves, perm */
+    char name[32];   /* stage: gemm, potf2, trsm_leaf, getf2_leaf, row_moves, perm,
+                        gemm_lowp, solve, batched, gemm_update (large-tile
+                        trailing-update GEMM) */
     int calls;       /* launches */
     double flops;    /* algorithmic flops of those launches */
     double bytes;    /* algorithmic HBM bytes of those launches */
@@ -84,6 +86,10 @@ typedef struct mifact_stage_stats {
 } mifact_stage_stats;
 
 mifact_status mifact_profile_begin(mifact_handle handle);
+/* Records only the stages whose bit is set in stage_mask (bit order: gemm,
+ * potf2, trsm_leaf, getf2_leaf, row_moves, perm, gemm_lowp, solve, batched,
+ * gemm_update), so a timed run can bracket just the dominant kernel. */
+mifact_status mifact_profile_begin_stages(mifact_handle handle, unsigned stage_mask);
 /* Writes up to max_stages entries; *count receives the number of stages. */
 mifact_status mifact_profile_end(mifact_handle handle, mifact_stage_stats* stats,
                                  int max_stages, int* count);
