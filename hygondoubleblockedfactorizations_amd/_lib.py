@@ -48,6 +48,9 @@ SIGNATURES = {
     "double_blocked_getrf_double": [P, P, I, I, I, I, I, I, P, P],
     "double_blocked_getrf_batched_pivot": [P, P, I, I, I, I, I, I, I, P, P, P],
     "double_blocked_getrf_double_batched_pivot": [P, P, I, I, I, I, I, I, I, P, P, P],
+    # getrf/double_blocked_getrf_npvt.h
+    "double_blocked_getrf_npvt": [P, P, I, I, I, I, P],
+    "double_blocked_getrf_npvt_double": [P, P, I, I, I, I, P],
     # gemm/update_gemm.h
     "gemm_nt_update": [P, I, I, I, P, I, P, I, P, I],
     "gemm_nt_update_double": [P, I, I, I, P, I, P, I, P, I],

@@ -151,6 +151,32 @@ void trsm_llu_iterative(mifact_handle h, int rows, int cols, const T* L, long lo
     }
 }
 
+// Unpivoted LU of an m x w panel (rows below the diagonal block included):
+// recursive halving at multiples of 64; each 64-column leaf factors its
+// diagonal block in one wave and forms the rows below as A21 inv(U11) with
+// an in-place MFMA multiply (inv(U11)^T from trtri_blocks).
+template <typename T>
+void getrf_npvt_panel(mifact_handle h, T* A, long long lda, int m, int w, int* info,
+                      int col_offset, T* inv)
+{
+    if (w <= 0 || m <= 0) return;
+    if (w <= 64) {
+        getf2_npvt_block<T>(h, w, A, lda, info, col_offset);
+        if (m > w) {
+            trtri_blocks<T>(h, w, A, lda, inv, false, true);
+            gemm_overwrite<T>(h, kOpNT, m - w, w, w, T(1), A + w, lda, inv, 64, A + w, lda);
+        }
+        return;
+    }
+    const int w1 = split_point(w, 64);
+    const int w2 = w - w1;
+    getrf_npvt_panel<T>(h, A, lda, m, w1, info, col_offset, inv);
+    trsm_llu<T>(h, w1, w2, A, lda, at(A, lda, 0, w1), lda);
+    gemm<T>(h, kOpNN, kShapeFull, m - w1, w2, w1, T(-1), at(A, lda, w1, 0), lda,
+            at(A, lda, 0, w1), lda, at(A, lda, w1, w1), lda);
+    getrf_npvt_panel<T>(h, at(A, lda, w1, w1), lda, m - w1, w2, info, col_offset + w1, inv);
+}
+
 // In-place lower Cholesky of an s x s diagonal block (any s): recursive
 // halving down to 64-wide one-wave leaves, which also leave the inverses of
 // their blocks at inv (local column c at inv + 64 c) for the panel solve.

@@ -17,6 +17,7 @@
 #include "blas3.hpp"
 #include "drivers.hpp"
 #include "getrf/double_blocked_getrf.h"
+#include "getrf/double_blocked_getrf_npvt.h"
 
 namespace mifact {
 namespace {
@@ -150,6 +151,36 @@ void double_blocked_getrf_impl(mifact_handle h, T* A, int lda_i, int n, int oute
     }
 }
 
+// Unpivoted double-blocked LU (src/getrf/double_blocked_getrf_npvt.cpp:94-148):
+// the same two-level schedule with rocsolver_sgetrf_npvt replaced by the
+// recursive no-pivot panel and no row exchanges; outer TRSM direct.
+template <typename T>
+void double_blocked_getrf_npvt_impl(mifact_handle h, T* A, int lda_i, int n, int nb, int b,
+                                    int* info)
+{
+    if (h == nullptr) {
+        report_status(nullptr, mifact_status_invalid_handle, "double_blocked_getrf_npvt: null handle");
+        return;
+    }
+    if (n <= 0) return;
+    if (nb <= 0) nb = n;
+    if (b <= 0) b = nb;
+    b = imin(b, nb);
+    const long long lda = lda_i;
+    if (!ensure_workspace(h, n)) return;
+    T* inv = invdiag<T>(h);
+    if (info != nullptr) clear_int(h, info);
+    for (int outer = 0; outer < n; outer += nb) {
+        const int outer_end = outer + imin(nb, n - outer);
+        for (int inner = outer; inner < outer_end; inner += b) {
+            const int isz = imin(b, outer_end - inner);
+            getrf_npvt_panel<T>(h, at(A, lda, inner, inner), lda, n - inner, isz, info, inner, inv);
+            inner_update<T>(h, A, lda, n, outer, inner + isz, outer_end, b, kPrecFP32);
+        }
+        outer_update<T>(h, A, lda, n, outer_end, nb, getrf_outer_trsm_direct, 0, kPrecFP32);
+    }
+}
+
 template void double_blocked_getrf_impl<float>(mifact_handle, float*, int, int, int, int,
                                               getrf_outer_trsm_mode, int, int*, int*, bool,
                                               float*, long long, LowPrecision);
@@ -234,6 +265,18 @@ void double_blocked_getrf_double_batched_pivot(mifact_handle handle, double* A, 
     (void)inner_gemm_m_tile;
     mifact::getrf_batched_entry<double>(handle, A, lda, n, outer_nb, inner_nb, outer_trsm_mode,
                                         trsm_block, ipiv, info, workspace);
+}
+
+void double_blocked_getrf_npvt(mifact_handle handle, float* A, int lda, int n, int nb, int b,
+                               int* info)
+{
+    mifact::double_blocked_getrf_npvt_impl<float>(handle, A, lda, n, nb, b, info);
+}
+
+void double_blocked_getrf_npvt_double(mifact_handle handle, double* A, int lda, int n, int nb,
+                                      int b, int* info)
+{
+    mifact::double_blocked_getrf_npvt_impl<double>(handle, A, lda, n, nb, b, info);
 }
 
 }  // extern "C"

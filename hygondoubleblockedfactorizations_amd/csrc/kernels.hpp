@@ -45,6 +45,11 @@ template <typename T>
 void potf2_base(mifact_handle h, int s, T* A, long long lda, T* inv, int* info,
                 int col_offset);
 
+// Unpivoted LU of an s x s block (s <= 64) in one wave; a zero pivot at
+// local column j sets *info = col_offset + j + 1 if it was 0.
+template <typename T>
+void getf2_npvt_block(mifact_handle h, int s, T* A, long long lda, int* info, int col_offset);
+
 // Inverses of the 64 x 64 diagonal blocks of the n x n lower triangle of L
 // (unit diagonal when unit): block b at inv + 4096 * b, ld 64.  upper: the
 // blocks of an upper-triangular U are inverted as U^T, giving inv(U)^T.

@@ -7,6 +7,8 @@
 // its inverted diagonal blocks); everything above the leaves is MFMA GEMM.
 #include <hip/hip_runtime.h>
 
+#include <limits>
+
 #include "kernels.hpp"
 
 namespace mifact {
@@ -93,6 +95,48 @@ __global__ __launch_bounds__(64) void potf2_kernel(int s, T* __restrict__ A, lon
     for (int j = 0; j < kLeaf; ++j)
         if (i < s && j < s) inv[j * kLeaf + i] = m[j];
 }
+
+// Unpivoted LU of an s x s block (s <= 64) in one wave: lane i owns row i
+// in LDS, right-looking column by column (LAPACK sgetf2 arithmetic without
+// the row search: multipliers by the reciprocal when |pivot| >= sfmin).  A
+// zero pivot records col_offset + j + 1 in *info (first in launch order)
+// and leaves its column unscaled.
+template <typename T>
+__global__ __launch_bounds__(64) void getf2_npvt_kernel(int s, T* A, long long lda, int* info,
+                                                        int col_offset)
+{
+    __shared__ T a[64][65];
+    const int lane = threadIdx.x;
+    for (int c = 0; c < s; ++c) a[lane][c] = lane < s ? A[off(lane, c, lda)] : T(0);
+    __syncthreads();
+    const T sfmin = std::numeric_limits<T>::min();
+    for (int j = 0; j < s; ++j) {
+        const T d = a[j][j];
+        if (d == T(0)) {
+            if (lane == 0) atomicCAS(info, 0, col_offset + j + 1);
+        } else if (lane > j && lane < s) {
+            const T l = (fabs(d) >= sfmin) ? a[lane][j] * (T(1) / d) : a[lane][j] / d;
+            a[lane][j] = l;
+            for (int c = j + 1; c < s; ++c) a[lane][c] -= l * a[j][c];
+        }
+        __syncthreads();
+    }
+    if (lane < s)
+        for (int c = 0; c < s; ++c) A[off(lane, c, lda)] = a[lane][c];
+}
+
+template <typename T>
+void getf2_npvt_block(mifact_handle h, int s, T* A, long long lda, int* info, int col_offset)
+{
+    if (s <= 0) return;
+    StageScope scope(h, kStagePotf2, 2.0 * s * s * s / 3.0, 2.0 * sizeof(T) * s * s);
+    hipLaunchKernelGGL(getf2_npvt_kernel<T>, dim3(1), dim3(64), 0, h->stream, s, A, lda, info,
+                       col_offset);
+    MIFACT_LAUNCHED(h);
+}
+
+template void getf2_npvt_block<float>(mifact_handle, int, float*, long long, int*, int);
+template void getf2_npvt_block<double>(mifact_handle, int, double*, long long, int*, int);
 
 template <typename T>
 void potf2_base(mifact_handle h, int s, T* A, long long lda, T* inv, int* info, int col_offset)
