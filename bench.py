@@ -554,6 +554,19 @@ def main():
         "batched": batched,
     }
     print(json.dumps(line), flush=True)
+    # raw timing CSV in the reference's layout when BENCHMARK_CSV_PATH is set
+    from hygondoubleblockedfactorizations_amd import report
+    rows = []
+    for r in routines:
+        rows.append({"method": f"mifact double_blocked_{r}",
+                     "parameters": f"nb={per[r]['nb']},b={per[r]['b']}",
+                     "times_ms": times[r], "flops": flops[r],
+                     "residual_metric": "gpu_fp64_full_relative_residual"
+                     if "residual_fp64" in per[r] else "none",
+                     "residual_value": per[r].get("residual_fp64", 0.0)})
+    path = report.write_timing_csv(f"{'/'.join(r.upper() for r in routines)} n={n} FP32", rows)
+    if path:
+        log(f"raw timing CSV: {path}")
 
 
 if __name__ == "__main__":
