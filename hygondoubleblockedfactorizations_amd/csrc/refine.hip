@@ -564,6 +564,7 @@ __global__ __launch_bounds__(256) void trsv_kernel(const float* F, long long ldf
                                                    unsigned long long ticket_base, int* status)
 {
     __shared__ float tile[64][65];
+    __shared__ float dinv[64][65];   // the diagonal block's inverse, loaded up front
     __shared__ float xs[64][NR];
     __shared__ float part[4][64][NR];
     __shared__ int bsh;
@@ -581,6 +582,13 @@ __global__ __launch_bounds__(256) void trsv_kernel(const float* F, long long ldf
     float acc[NR];
 #pragma unroll
     for (int k = 0; k < NR; ++k) acc[k] = 0.0f;
+    // the inverse does not depend on other blocks: fetch it before waiting
+#pragma unroll
+    for (int e = 0; e < 16; ++e) {
+        const int idx = tid + 256 * e;
+        const int lr = idx & 63, lc = idx >> 6;
+        dinv[lc][lr] = inv[static_cast<size_t>(bi) * 4096 + lc * 64 + lr];
+    }
 
     const int ndep = MODE == kFwdLower ? bi : nblk - 1 - bi;
     for (int q = 0; q < ndep; ++q) {
@@ -628,12 +636,6 @@ __global__ __launch_bounds__(256) void trsv_kernel(const float* F, long long ldf
     // s = b_i - sum; x_i = D_i^{-1} s with the stored inverse block
 #pragma unroll
     for (int k = 0; k < NR; ++k) part[w][lane][k] = acc[k];
-#pragma unroll
-    for (int e = 0; e < 16; ++e) {
-        const int idx = tid + 256 * e;
-        const int lr = idx & 63, lc = idx >> 6;
-        tile[lc][lr] = inv[static_cast<size_t>(bi) * 4096 + lc * 64 + lr];
-    }
     __syncthreads();
     for (int e = tid; e < 64 * NR; e += 256) {
         const int r = e / NR, k = e % NR;
@@ -652,7 +654,7 @@ __global__ __launch_bounds__(256) void trsv_kernel(const float* F, long long ldf
 #pragma unroll 4
     for (int cc = 0; cc < 16; ++cc) {
         const int c = w * 16 + cc;
-        const float d = (MODE == kFwdLower) ? tile[c][lane] : tile[lane][c];
+        const float d = (MODE == kFwdLower) ? dinv[c][lane] : dinv[lane][c];
 #pragma unroll
         for (int k = 0; k < NR; ++k) y[k] = fmaf(d, xs[c][k], y[k]);
     }
