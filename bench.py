@@ -128,7 +128,9 @@ def potrf_residual_gpu(torch, A_orig, L_work):
     return float(torch.linalg.norm(R) / torch.linalg.norm(M))
 
 
-def getrf_residual_gpu(torch, A_orig, LU_work, ipiv):
+def getrf_residual_gpu(torch, A_orig, LU_work, ipiv, bs=8192):
+    """||P A - L U||_F / ||A||_F in FP64, column block by column block of
+    (L U) = sum over k-blocks of L(:, kb) U(kb, cols) (memory O(n bs))."""
     import numpy as np
     n = A_orig.shape[0]
     perm = np.arange(n)
@@ -137,14 +139,30 @@ def getrf_residual_gpu(torch, A_orig, LU_work, ipiv):
         p = piv[k]
         if p != k:
             perm[k], perm[p] = perm[p], perm[k]
-    M = A_orig.T.double()
-    PA = M[torch.from_numpy(perm).cuda()]
-    X = LU_work.T.double()
-    L = torch.tril(X, -1)
-    L.diagonal().add_(1.0)
-    U = torch.triu(X)
-    R = PA - L @ U
-    return float(torch.linalg.norm(R) / torch.linalg.norm(M))
+    permd = torch.from_numpy(perm).cuda()
+    X = LU_work.T          # logical factors (FP32 view, no copy)
+    M = A_orig.T
+    num = 0.0
+    den = 0.0
+    for c0 in range(0, n, bs):
+        c1 = min(n, c0 + bs)
+        acc = M[permd, c0:c1].double()
+        den += float((M[:, c0:c1].double() ** 2).sum())
+        for k0 in range(0, c1, bs):
+            k1 = min(c1, k0 + bs)
+            Lk = X[:, k0:k1].double()
+            rows = torch.arange(n, device=Lk.device).unsqueeze(1)
+            cols = torch.arange(k0, k1, device=Lk.device).unsqueeze(0)
+            Lk = torch.where(rows > cols, Lk, torch.where(rows == cols, 1.0, 0.0))
+            Uk = X[k0:k1, c0:c1].double()
+            ur = torch.arange(k0, k1, device=Uk.device).unsqueeze(1)
+            uc = torch.arange(c0, c1, device=Uk.device).unsqueeze(0)
+            Uk = torch.where(ur <= uc, Uk, 0.0)
+            acc -= Lk @ Uk
+            del Lk, Uk
+        num += float((acc ** 2).sum())
+        del acc
+    return float(np.sqrt(num / den))
 
 
 def cpu_baseline(torch, args):
