@@ -31,12 +31,12 @@ import time
 
 METRIC = "POTRF/GETRF TFLOP/s @n=32768; mixed+IR vs FP32 solve time, same accuracy"
 FP32_PEAK_TFLOPS = 157.3      # MI355X dense FP32 (MFMA)
-# HBM bytes per dispatch of the FP32 lower-trapezoid update GEMM of POTRF at
-# (n, nb, b): rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE runs of this script
-# (POTRF only), scaled by the same counters' reading of the bench's own
-# n^2 x 4 B input copy (factors 2.0 and 1.0); see
-# profiles/gemm_update_potrf_n32768_nb8192_b256_pmc.txt.
-TRAPEZOID_GEMM_TRAFFIC = {(32768, 8192, 256): 4.2212e10}
+# HBM bytes per dispatch of the large-tile update GEMM, averaged over its
+# dispatches in one bench step, keyed by (n, potrf nb, potrf b, getrf nb,
+# getrf b): calibrated from rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE runs of
+# this script, scaled by the same counters' reading of a copy of known size.
+GEMM_TRAFFIC = {}
+GEMM_TRAFFIC_SOURCE = ""
 HBM_PEAK_GBS = 8000.0
 
 
@@ -57,8 +57,11 @@ def parse():
     ap.add_argument("--trsm-block", type=int, default=0)
     ap.add_argument("--routines", default="potrf,getrf")
     ap.add_argument("--seed", type=int, default=20260716)
-    ap.add_argument("--cpu-n", type=int, default=16384)
-    ap.add_argument("--cpu-threads", type=int, default=0)
+    ap.add_argument("--cpu-n", type=int, default=0,
+                    help="order of the CPU LAPACK sample (default: the GPU step's n)")
+    ap.add_argument("--cpu-threads", type=int, default=0,
+                    help="LAPACK threads (default: the GPU's NUMA-node cores in this "
+                         "process's affinity, capped by OMP_NUM_THREADS)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-verify", action="store_true")
     ap.add_argument("--no-profile", action="store_true")
@@ -165,25 +168,63 @@ def getrf_residual_gpu(torch, A_orig, LU_work, ipiv, bs=8192):
     return float(np.sqrt(num / den))
 
 
-def cpu_baseline(torch, args):
-    """LAPACK spotrf + sgetrf on host cores, bounded sample (n = cpu_n)."""
+def gpu_numa_cores():
+    """Host cores of the GPU's NUMA node that this process may run on, and
+    the node id (all affinity cores and -1 when the node is unknown)."""
+    aff = sorted(os.sched_getaffinity(0))
+    node = -1
+    try:
+        import glob
+        for dev in sorted(glob.glob("/sys/class/drm/card*/device")):
+            vendor = open(os.path.join(dev, "vendor")).read().strip()
+            if vendor != "0x1002":
+                continue
+            node = int(open(os.path.join(dev, "numa_node")).read().strip())
+            break
+    except Exception:
+        node = -1
+    if node < 0:
+        return aff, -1
+    cpus = set()
+    try:
+        for part in open(f"/sys/devices/system/node/node{node}/cpulist").read().strip().split(","):
+            if "-" in part:
+                a, b = part.split("-")
+                cpus.update(range(int(a), int(b) + 1))
+            elif part:
+                cpus.add(int(part))
+    except Exception:
+        return aff, node
+    both = [c for c in aff if c in cpus]
+    return (both or aff), node
+
+
+def cpu_baseline(torch, args, n_default):
+    """LAPACK spotrf + sgetrf on host cores: the same two input families and
+    order as the GPU step (the bytes are regenerated from the same seeds on
+    the GPU and copied to the host), one run each."""
     import numpy as np
     from oracle import oracle as O
     path, config = O.lapack_open()
-    threads = args.cpu_threads or min(16, len(os.sched_getaffinity(0)))
+    cores, node = gpu_numa_cores()
+    threads = args.cpu_threads or len(cores)
+    cap = int(os.environ.get("OMP_NUM_THREADS", "0") or 0)
+    if not args.cpu_threads and cap > 0:
+        threads = min(threads, cap)
     O.lapack_set_threads(threads)
-    n = args.cpu_n
-    A = gen_spd(torch, n, args.seed).cpu().numpy()          # symmetric: layout-free
-    B = gen_uniform(torch, n, args.seed + 1).cpu().numpy().T  # column-major view
-    A = np.asfortranarray(A)
-    B = np.asfortranarray(B)
+    n = args.cpu_n or n_default
+    A = np.asfortranarray(gen_spd(torch, n, args.seed).cpu().numpy())  # symmetric
+    B = np.asfortranarray(gen_uniform(torch, n, args.seed + 1).cpu().numpy().T)
+    torch.cuda.empty_cache()
     # warm the library (threads, buffers) on a small case
     O.spotrf(np.asfortranarray(A[:1024, :1024]))
+    log(f"cpu baseline: n={n}, {threads} threads")
     t0 = time.perf_counter()
     info = O.spotrf(A)
     t1 = time.perf_counter()
     ipiv, info2 = O.sgetrf(B)
     t2 = time.perf_counter()
+    del A, B
     flops = n ** 3 / 3.0 + 2.0 * n ** 3 / 3.0
     return {
         "value": flops / (t2 - t0) / 1e12,
@@ -191,9 +232,12 @@ def cpu_baseline(torch, args):
         "cores": threads,
         "kind": "port",
         "sample": (f"LAPACK spotrf + sgetrf via {os.path.basename(path)} ({config.strip()}), "
-                   f"n={n} FP32 of the same input families (seeded on the GPU, copied to host), "
-                   f"{threads} threads; spotrf {t1 - t0:.2f} s, sgetrf {t2 - t1:.2f} s; "
+                   f"n={n} FP32, the same input families and seeds as the GPU step, "
+                   f"{threads} threads (GPU NUMA node {node}: {len(cores)} cores in this "
+                   f"process's affinity); spotrf {t1 - t0:.2f} s, sgetrf {t2 - t1:.2f} s; "
                    f"info {info}/{info2}"),
+        "numa_node": node,
+        "numa_cores": len(cores),
         "potrf_tflops": n ** 3 / 3.0 / (t1 - t0) / 1e12,
         "getrf_tflops": 2.0 * n ** 3 / 3.0 / (t2 - t1) / 1e12,
     }
@@ -507,7 +551,12 @@ def main():
                     "avg_launch_ms": g["ms"] / max(g["calls"], 1),
                     "flops_per_launch": g["flops"] / max(g["calls"], 1)}
     if roofline is not None:
-        roofline["traffic"] = TRAPEZOID_GEMM_TRAFFIC.get((n, pnb, pb))
+        # calibrated, not live: rocprofv3 --pmc runs of this bench command;
+        # null for a block configuration that was not measured
+        key = (n, pnb, pb, gnb, gb)
+        roofline["traffic"] = GEMM_TRAFFIC.get(key)
+        roofline["traffic_source"] = (GEMM_TRAFFIC_SOURCE if key in GEMM_TRAFFIC
+                                      else "not measured for this block configuration")
     stage_out = {k: {"calls": v["calls"], "ms": v["ms"],
                      "tflops": (v["flops"] / (v["ms"] * 1e-3) / 1e12) if v["ms"] > 0 else None,
                      "gbs": (v["bytes"] / (v["ms"] * 1e-3) / 1e9) if v["ms"] > 0 else None}
@@ -544,7 +593,7 @@ def main():
     cpu = None
     if not args.no_cpu_baseline:
         try:
-            cpu = cpu_baseline(torch, args)
+            cpu = cpu_baseline(torch, args, n)
         except Exception as e:  # report, never hide the GPU line
             log(f"cpu baseline failed: {e}")
 

@@ -26,6 +26,7 @@ SIGNATURES = {
     "mifact_get_last_error": [P],
     "mifact_reserve_workspace": [P, I],
     "mifact_set_gemm_config": [P, I],
+    "mifact_set_wait_limit": [P, ctypes.c_longlong],
     "mifact_version": [],
     "mifact_profile_begin": [P],
     "mifact_profile_begin_stages": [P, ctypes.c_uint],
@@ -106,6 +107,7 @@ _RESTYPE = {
     "mifact_get_last_error": I,
     "mifact_reserve_workspace": I,
     "mifact_set_gemm_config": I,
+    "mifact_set_wait_limit": I,
     "mifact_version": ctypes.c_char_p,
     "mifact_profile_begin": I,
     "mifact_profile_begin_stages": I,

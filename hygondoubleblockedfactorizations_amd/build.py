@@ -14,6 +14,7 @@ import os
 import shutil
 import subprocess
 import sys
+import tempfile
 
 PKG = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(PKG)
@@ -45,13 +46,67 @@ COMMON_FLAGS = [
 ]
 
 
+LLVM_BIN = os.path.join(os.environ.get("ROCM_PATH", "/opt/rocm"), "llvm", "bin")
+
+
+def _device_kernels(obj: str) -> set:
+    """Kernel symbols of the gfx950 code object bundled in `obj`."""
+    with tempfile.TemporaryDirectory() as tmp:
+        fat = os.path.join(tmp, "fatbin")
+        co = os.path.join(tmp, "device.co")
+        subprocess.run(["objcopy", "-O", "binary", "--only-section=.hip_fatbin", obj, fat],
+                       check=True, capture_output=True)
+        subprocess.run([os.path.join(LLVM_BIN, "clang-offload-bundler"), "--unbundle",
+                        "--type=o", f"--input={fat}",
+                        f"--targets=hipv4-amdgcn-amd-amdhsa--{ARCH}", f"--output={co}"],
+                       check=True, capture_output=True)
+        out = subprocess.run([os.path.join(LLVM_BIN, "llvm-readelf"), "-s", co], check=True,
+                             capture_output=True, text=True).stdout
+    names = set()
+    for line in out.splitlines():
+        f = line.split()
+        if len(f) >= 8 and f[3] == "FUNC" and f[4] == "GLOBAL":
+            names.add(f[7])
+    return names
+
+
+def _host_kernels(obj: str) -> set:
+    """Kernel names the host half of `obj` registers with the HIP runtime
+    (the mangled names in its string table)."""
+    with tempfile.TemporaryDirectory() as tmp:
+        out = os.path.join(tmp, "str")
+        subprocess.run(["objcopy", "-O", "binary", "--only-section=.rodata.str1.1", obj, out],
+                       check=True, capture_output=True)
+        data = open(out, "rb").read()
+    return {w.decode() for w in data.split(b"\0") if w.startswith(b"_ZN6mifact")}
+
+
+def verify_object(obj: str) -> None:
+    """Every kernel the host code registers must exist in the device code
+    object of the same file: a launch of a missing one aborts the process
+    at run time ('Cannot find Symbol')."""
+    host = _host_kernels(obj)
+    if not host:
+        return
+    dev = _device_kernels(obj)
+    missing = sorted(host - dev)
+    if missing:
+        raise RuntimeError(f"{obj}: host registers kernels absent from its device code "
+                           f"(stale or mixed object): {missing[:4]}")
+
+
 def _compile(src: str) -> str:
     path = os.path.join(CSRC, src)
     obj = os.path.join(BUILD, os.path.splitext(src)[0] + ".o")
-    cmd = [HIPCC, *COMMON_FLAGS, "-x", "hip", "-c", path, "-o", obj]
+    # write to a private name, check, then rename: no other writer of the
+    # same path can leave a mixed object behind
+    tmp_obj = f"{obj}.{os.getpid()}.tmp"
+    cmd = [HIPCC, *COMMON_FLAGS, "-x", "hip", "-c", path, "-o", tmp_obj]
     res = subprocess.run(cmd, capture_output=True, text=True)
     if res.returncode != 0:
         raise RuntimeError(f"hipcc failed for {src}:\n{res.stdout}\n{res.stderr}")
+    verify_object(tmp_obj)
+    os.replace(tmp_obj, obj)
     # hipcc's host pass does not know the device-only feature: drop that note.
     noise = "is not a recognized feature for this target"
     msgs = [l for l in res.stderr.splitlines() if noise not in l]

@@ -199,16 +199,35 @@ void potrf_diag(mifact_handle h, int s, T* A, long long lda, int* info, int col_
                   inv + static_cast<size_t>(s1) * 64);
 }
 
+inline bool is_pow2(int v) { return v > 0 && (v & (v - 1)) == 0; }
+
 // Recursive partial-pivot LU of rows [c0, n) x columns [c0, c0 + w) of A.
 // ipiv is written 1-based and global; orig tracks row contents; inv holds
 // kPermLevels snapshot arrays of n ints (level 0 belongs to the caller).
+//
+// Leaves (w <= 32) leave behind the move list of their own exchanges and
+// the inverse of their unit-lower diagonal block, so a node whose left or
+// right child is a leaf needs no snapshot, and its triangular solve needs
+// no separate inversion: a node of width exactly 64 pairs its two leaves'
+// 32 x 32 inverses into the 64 x 64 inverse at pinv + 64 c0 (first leaf on
+// the top-left, second on the bottom-right, inv_pair fills the bottom-left
+// once the block's rows are final).  A left child of power-of-two width
+// >= 64 is made of such 64-nodes, so its solve uses those blocks too; any
+// other width forms its inverses with trtri (the generic trsm_llu).
+// role: 0 lone, 1 first leaf of a 64-node, 2 second leaf of a 64-node.
 template <typename T>
 void getrf_panel_rec(mifact_handle h, T* A, long long lda, int n, int c0, int w, int* ipiv,
-                     int* orig, int* inv, int nws, int* info, int level)
+                     int* orig, int* inv, int nws, int* info, int level, T* pinv, int role = 0)
 {
     if (w <= 0) return;
+    Workspace ws = workspace(h);
+    int* mv_src = ws.leaf_moves;
+    int* mv_dst = ws.leaf_moves + 64;
     if (w <= 32) {
-        getf2_leaf<T>(h, A, lda, n, c0, w, ipiv, orig, info);
+        T* ib = (role == 2) ? pinv + 64 * static_cast<size_t>(c0 - 32) + 32 + 64 * 32
+                            : pinv + 64 * static_cast<size_t>(c0);
+        T* iz = (role == 1) ? pinv + 64 * static_cast<size_t>(c0) + 64 * 32 : nullptr;
+        getf2_leaf<T>(h, A, lda, n, c0, w, ipiv, orig, info, ib, iz, mv_src, mv_dst);
         return;
     }
     if (level >= kPermLevels - 2) {
@@ -217,16 +236,40 @@ void getrf_panel_rec(mifact_handle h, T* A, long long lda, int n, int c0, int w,
     }
     const int w1 = split_point(w, 32);
     const int w2 = w - w1;
+    const bool pair = (w == 64);
     int* inv_l = inv + static_cast<size_t>(level) * nws;
-    perm_snapshot(h, orig, inv_l, c0, n);
-    getrf_panel_rec<T>(h, A, lda, n, c0, w1, ipiv, orig, inv, nws, info, level + 1);
-    apply_row_moves<T>(h, A, lda, c0 + w1, c0 + w, c0, w1, ipiv, orig, inv_l, nullptr, 0);
-    trsm_llu<T>(h, w1, w2, at(A, lda, c0, c0), lda, at(A, lda, c0, c0 + w1), lda);
+    if (w1 > 32) perm_snapshot(h, orig, inv_l, c0, n);
+    getrf_panel_rec<T>(h, A, lda, n, c0, w1, ipiv, orig, inv, nws, info, level + 1, pinv,
+                       pair ? 1 : 0);
+    if (w1 > 32) {
+        apply_row_moves<T>(h, A, lda, c0 + w1, c0 + w, c0, w1, ipiv, orig, inv_l, nullptr, 0);
+    } else {
+        apply_moves<T>(h, A, lda, c0 + w1, c0 + w, 2 * w1, mv_src, mv_dst, nullptr, 0);
+    }
+    if (w1 <= 32 || is_pow2(w1)) {
+        trsm_llu_inv<T>(h, w1, w2, at(A, lda, c0, c0), lda, at(A, lda, c0, c0 + w1), lda,
+                        pinv + 64 * static_cast<size_t>(c0));
+    } else {
+        trsm_llu<T>(h, w1, w2, at(A, lda, c0, c0), lda, at(A, lda, c0, c0 + w1), lda);
+    }
     gemm<T>(h, kOpNN, kShapeFull, n - c0 - w1, w2, w1, T(-1), at(A, lda, c0 + w1, c0), lda,
             at(A, lda, c0, c0 + w1), lda, at(A, lda, c0 + w1, c0 + w1), lda);
-    perm_snapshot(h, orig, inv_l, c0 + w1, n);
-    getrf_panel_rec<T>(h, A, lda, n, c0 + w1, w2, ipiv, orig, inv, nws, info, level + 1);
-    apply_row_moves<T>(h, A, lda, c0, c0 + w1, c0 + w1, w2, ipiv, orig, inv_l, nullptr, 0);
+    if (w2 > 32) perm_snapshot(h, orig, inv_l, c0 + w1, n);
+    getrf_panel_rec<T>(h, A, lda, n, c0 + w1, w2, ipiv, orig, inv, nws, info, level + 1, pinv,
+                       pair ? 2 : 0);
+    if (w2 > 32) {
+        apply_row_moves<T>(h, A, lda, c0, c0 + w1, c0 + w1, w2, ipiv, orig, inv_l, nullptr, 0);
+    } else {
+        apply_moves<T>(h, A, lda, c0, c0 + w1, 2 * w2, mv_src, mv_dst, nullptr, 0);
+    }
+    if (pair) inv_pair<T>(h, at(A, lda, c0, c0), lda, pinv + 64 * static_cast<size_t>(c0));
+}
+
+// The panel-inverse region of the workspace (see getrf_panel_rec).
+template <typename T>
+T* panel_inv(mifact_handle h)
+{
+    return static_cast<T*>(workspace(h).panel_inv);
 }
 
 }  // namespace mifact

@@ -59,8 +59,12 @@ struct mifact_handle_s {
     unsigned long long barrier_base = 0;
     // Epoch of the next GETF2 leaf column (exchange records carry it).
     unsigned int leaf_epoch = 1;
-    // Device status word written by kernels whose bounded waits expired.
+    // Device status word written by kernels whose bounded waits expired
+    // (cleared at the start of every pivoting driver call and by
+    // mifact_get_last_error).
     int* kernel_status = nullptr;
+    // Polls after which a bounded wait gives up (0 = each kernel's default).
+    long long wait_limit = 0;
 
     // Solver arena (triangular solves, residuals, refinement) and the pinned
     // host buffer its convergence tests read back.
@@ -145,12 +149,19 @@ struct Workspace {
     int* inv = nullptr;          // kPermLevels x n ints: snapshots of orig^-1
     int* moves = nullptr;        // 4 x n ints: move lists (src 2n | dst 2n)
     float* exchange = nullptr;   // cooperative panel exchange records
-    void* invdiag = nullptr;     // n x 64 elements (double-sized): inverses of
-                                 // the 64 x 64 diagonal blocks of a solve
+    void* invdiag = nullptr;     // (n + 64) x 64 elements (double-sized): inverses
+                                 // of the 64 x 64 diagonal blocks of a solve
+    void* panel_inv = nullptr;   // (n + 64) x 64 elements (double-sized): the
+                                 // GETRF panel's diagonal-block inverses, the
+                                 // block at global column c at offset 64 c
+    int* leaf_moves = nullptr;   // move list of the last GETF2 leaf (2 x 64 ints)
+    void* stage = nullptr;       // row-move staging: 2n x kStageCols doubles
+    size_t stage_bytes = 0;
 };
 
 constexpr int kPermLevels = 24;             // panel/outer + recursion levels
 constexpr size_t kExchangeBytes = 1 << 20;  // 2 parities x (G+1) records
+constexpr int kStageCols = 16;              // columns per row-move staging chunk (FP64)
 
 bool ensure_workspace(mifact_handle h, int n);
 Workspace workspace(mifact_handle h);

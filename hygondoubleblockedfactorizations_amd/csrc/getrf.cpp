@@ -106,6 +106,7 @@ void double_blocked_getrf_impl(mifact_handle h, T* A, int lda_i, int n, int oute
     int* inv_outer = ws.inv + kLevelOuter * nws;
     int* inv_all = ws.inv + kLevelAll * nws;
     if (info != nullptr) clear_int(h, info);
+    clear_int(h, h->kernel_status);
     perm_identity(h, orig, n);
     if (batched) perm_identity(h, inv_all, n);
 
@@ -119,7 +120,7 @@ void double_blocked_getrf_impl(mifact_handle h, T* A, int lda_i, int n, int oute
             // panel GETRF (rocsolver_sgetrf at double_blocked_getrf.cpp:156-163)
             perm_snapshot(h, orig, inv_panel, inner, n);
             getrf_panel_rec<T>(h, A, lda, n, inner, isz, ipiv, orig, ws.inv,
-                               static_cast<int>(nws), info, kLevelPanel + 1);
+                               static_cast<int>(nws), info, kLevelPanel + 1, panel_inv<T>(h));
             if (!batched) {
                 // apply_panel_pivots (:165-166): every column outside the panel
                 apply_row_moves<T>(h, A, lda, 0, inner, inner, isz, ipiv, orig, inv_panel,

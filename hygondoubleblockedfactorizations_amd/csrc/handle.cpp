@@ -6,6 +6,7 @@
 #include <new>
 
 #include "common.hpp"
+#include "kernels.hpp"
 
 namespace mifact {
 
@@ -45,11 +46,15 @@ static size_t int_slots(size_t n) { return (1 + kPermLevels + 4) * n; }
 
 static size_t invdiag_bytes(size_t n) { return (n + 64) * 64 * sizeof(double); }
 
+// Staging of row moves whose move list does not fit in LDS: 2n entries of
+// kStageCols FP64 columns (2 kStageCols FP32 columns).
+static size_t stage_bytes(size_t n) { return 2 * n * kStageCols * sizeof(double); }
+
 static size_t workspace_bytes(int n)
 {
     const size_t ints = int_slots(static_cast<size_t>(n));
     return ((ints * sizeof(int) + 255) & ~static_cast<size_t>(255)) + kExchangeBytes
-         + invdiag_bytes(static_cast<size_t>(n));
+         + 2 * invdiag_bytes(static_cast<size_t>(n)) + stage_bytes(static_cast<size_t>(n));
 }
 
 bool ensure_workspace(mifact_handle h, int n)
@@ -97,6 +102,11 @@ Workspace workspace(mifact_handle h)
     const size_t off_bytes = (int_slots(n) * sizeof(int) + 255) & ~static_cast<size_t>(255);
     w.exchange = reinterpret_cast<float*>(base + off_bytes);
     w.invdiag = base + off_bytes + kExchangeBytes;
+    w.panel_inv = static_cast<char*>(w.invdiag) + invdiag_bytes(n);
+    w.stage = static_cast<char*>(w.panel_inv) + invdiag_bytes(n);
+    // the upper half of the exchange area holds the last leaf's move list
+    w.leaf_moves = reinterpret_cast<int*>(base + off_bytes + kExchangeBytes / 2);
+    w.stage_bytes = stage_bytes(n);
     return w;
 }
 
@@ -290,8 +300,11 @@ mifact_status mifact_get_last_error(mifact_handle h)
     if (hipMemcpyAsync(&kst, h->kernel_status, sizeof(int), hipMemcpyDeviceToHost,
                        h->stream) == hipSuccess
         && hipStreamSynchronize(h->stream) == hipSuccess) {
-        if (kst != 0 && s == mifact_status_success) {
-            s = mifact_status_kernel_timeout;
+        if (kst != 0) {
+            if (s == mifact_status_success) s = mifact_status_kernel_timeout;
+            // read and clear: later panel kernels on this handle run again
+            mifact::clear_int(h, h->kernel_status);
+            (void)hipStreamSynchronize(h->stream);
         }
     }
     h->last_error = mifact_status_success;
@@ -315,6 +328,14 @@ mifact_status mifact_set_gemm_config(mifact_handle h, int config)
     if (h == nullptr) return mifact_status_invalid_handle;
     if (config < 0 || config > 4) return mifact_status_invalid_value;
     h->gemm_config = config;
+    return mifact_status_success;
+}
+
+mifact_status mifact_set_wait_limit(mifact_handle h, long long polls)
+{
+    if (h == nullptr) return mifact_status_invalid_handle;
+    if (polls < 0) return mifact_status_invalid_value;
+    h->wait_limit = polls;
     return mifact_status_success;
 }
 

@@ -60,10 +60,22 @@ void trtri_blocks(mifact_handle h, int n, const T* L, long long ldl, T* inv, boo
 // Cooperative partial-pivot LU of a leaf panel: rows [c0, n) x columns
 // [c0, c0 + w) of the full matrix (w <= 32).  Writes ipiv[c0 + j] (global,
 // 1-based), carries orig[] (the origin of each row's content) along with the
-// row exchanges and records the first zero pivot in *info.
+// row exchanges and records the first zero pivot in *info.  Optionally:
+//   inv      receives the inverse of the leaf's unit-lower w x w diagonal
+//            block (32 x 32, ld 64, identity rows past w); inv_zero, when
+//            set, names a 32 x 32 block (ld 64) that is cleared;
+//   mv_src,  receive the move list of the leaf's own exchanges (2w entries,
+//   mv_dst   global rows, src -1 for "no move"), the same list
+//            apply_row_moves would derive from a snapshot taken before it.
 template <typename T>
 void getf2_leaf(mifact_handle h, T* A, long long lda, int n, int c0, int w,
-                int* ipiv, int* orig, int* info);
+                int* ipiv, int* orig, int* info, T* inv = nullptr, T* inv_zero = nullptr,
+                int* mv_src = nullptr, int* mv_dst = nullptr);
+
+// Completes the 64 x 64 inverse at inv (ld 64) of the unit-lower block
+// L(0:64, 0:64) whose two 32 x 32 diagonal inverses are already in place.
+template <typename T>
+void inv_pair(mifact_handle h, const T* L, long long ldl, T* inv);
 
 // inv[orig[p]] = p for p in [r0, n): snapshot of the row-origin map.
 void perm_snapshot(mifact_handle h, const int* orig, int* inv, int r0, int n);
@@ -72,12 +84,19 @@ void perm_identity(mifact_handle h, int* orig, int n);
 // *info = 0 with a kernel (no memset nodes in captured graphs).
 void clear_int(mifact_handle h, int* p);
 
+// Applies a move list to columns [c_begin, c_end) of A: for each entry t
+// with src[t] >= 0, row dst[t] receives the (pre-call) entry of row src[t].
+// tmp: optional global staging (>= nm elements per column chunk), used when
+// nm values per column do not fit in LDS; else the workspace's staging.
+template <typename T>
+void apply_moves(mifact_handle h, T* A, long long lda, int c_begin, int c_end, int nm,
+                 const int* src, const int* dst, T* tmp, long long tmp_elems);
+
 // Applies the rows exchanged since a snapshot to a block of columns.
 // Candidate destination rows: [p0, p0 + K) and ipiv[p0 .. p0+K) - 1.
 // For each candidate pos: src = inv[orig[pos]]; if src != pos the column
 // entry at pos receives the (pre-call) entry at src.  Columns [c_begin,
-// c_end) of A.  tmp: optional global staging (2K x column chunk), used when
-// 2K values per column do not fit in LDS.
+// c_end) of A.  tmp as for apply_moves.
 template <typename T>
 void apply_row_moves(mifact_handle h, T* A, long long lda, int c_begin,
                      int c_end, int p0, int K, const int* ipiv,

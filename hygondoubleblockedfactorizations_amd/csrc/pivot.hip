@@ -33,7 +33,8 @@
 namespace mifact {
 
 constexpr int kLeafW = 32;
-constexpr int kMaxG = 128;  // workgroups of a leaf (rows <= 128 * 256 * 4)
+constexpr int kMaxG = 128;           // workgroups of a leaf (rows <= 128 * 256 * 4)
+constexpr int kIdxNone = 0x7fffffff; // "no candidate" row index
 
 template <typename T> __device__ inline T sfmin();
 template <> __device__ inline float sfmin<float>() { return FLT_MIN; }
@@ -103,25 +104,31 @@ __device__ inline void wave_argmax(T& v, int& i)
     i = __builtin_amdgcn_readlane(i, 63);
 }
 
-// Exchange records.  For column j every workgroup publishes one record
-// (its pivot candidate: row index, |value|, the whole candidate row and the
-// row's origin tag) and workgroup 0 also publishes row j.  A record is RW
-// 64-bit words {payload: 32 bits, epoch: 32 bits}, each written by a single
-// coherent (sc1) 64-bit store, so a reader that sees column j's epoch in a
-// word also sees that word's payload.  Readers poll the records themselves
-// until every word carries the epoch: no counter, no fences, one round trip
-// when the data is already there.  Records are double-buffered by column
-// parity: a workgroup can only overwrite a parity slot (column j + 2) after
-// it has read every record of column j + 1, which every workgroup publishes
+// Exchange records of one leaf column, double-buffered by column parity.
+// Every word is 64 bits {payload: 32 bits, epoch: 32 bits} written by one
+// coherent 64-bit store, so a reader that sees the column's epoch in a word
+// also sees that word's payload; readers poll the words they need until all
+// carry the epoch (no counter, no fence).  Per parity:
+//   S[g]  small record of workgroup g: its candidate row and |value|;
+//   R[g]  the candidate row itself (kLeafW values) and its origin tag;
+//   J     row j (written by workgroup 0, read only by the pivot's owner).
+// Every workgroup polls the G small records, picks the pivot, then reads one
+// row record: G * SW + RW words per column instead of every candidate row.
+// A workgroup can only overwrite a parity slot (column j + 2) after it has
+// read every small record of column j + 1, which each workgroup publishes
 // only after it has finished reading column j.
 template <typename T>
 struct Rec {
-    static constexpr int PW = sizeof(T) / 4;           // payload words per T
-    static constexpr int IDX = 0;                       // candidate row
-    static constexpr int VAL = 1;                       // |value|
-    static constexpr int ROW = 1 + PW;                  // kLeafW values
-    static constexpr int TAG = ROW + PW * kLeafW;       // row origin
-    static constexpr int RW = TAG + 1;                  // words per record
+    static constexpr int PW = sizeof(T) / 4;       // payload words per T
+    static constexpr int SIDX = 0;
+    static constexpr int SVAL = 1;
+    static constexpr int SW = 1 + PW;
+    static constexpr int RTAG = PW * kLeafW;
+    static constexpr int RW = RTAG + 1;
+    static constexpr int S_OFF = 0;
+    static constexpr int R_OFF = kMaxG * SW;
+    static constexpr int J_OFF = R_OFF + kMaxG * RW;
+    static constexpr int PARITY = J_OFF + RW;
 };
 
 template <typename T>
@@ -135,10 +142,15 @@ struct LeafArgs {
     int* ipiv;
     int* orig;
     int* info;
-    unsigned long long* xrec;  // [2][kMaxG + 1][RW]
-    unsigned int epoch0;       // epoch of column j is epoch0 + j
+    unsigned long long* xrec;       // [2][Rec<T>::PARITY]
+    unsigned int epoch0;            // epoch of column j is epoch0 + j
     int* status;
+    unsigned long long spin_limit;  // unsuccessful polls before giving up
     int G;
+    T* inv;                         // inverse of the leaf's unit-lower block (ld 64) or null
+    T* inv_zero;                    // 32 x 32 block (ld 64) to clear, or null
+    int* mv_src;                    // move list of the leaf's pivots (2w entries) or null
+    int* mv_dst;
 };
 
 __device__ inline void xstore64(unsigned long long* p, unsigned int payload, unsigned int epoch)
@@ -171,24 +183,36 @@ template <> __device__ inline double from_words<double>(const unsigned int* w)
     return __longlong_as_double(static_cast<long long>(b));
 }
 
-// Writes one record (one thread).
+// One thread writes a small record.
 template <typename T>
-__device__ inline void publish(unsigned long long* rec, unsigned int epoch, int idx, T val,
-                               const T (&row)[kLeafW], int tag)
+__device__ inline void publish_small(unsigned long long* rec, unsigned int epoch, int idx, T val)
 {
     using R = Rec<T>;
     unsigned int w[R::PW];
-    xstore64(rec + R::IDX, static_cast<unsigned int>(idx), epoch);
     to_words(val, w);
 #pragma unroll
-    for (int q = 0; q < R::PW; ++q) xstore64(rec + R::VAL + q, w[q], epoch);
+    for (int q = 0; q < R::PW; ++q) xstore64(rec + R::SVAL + q, w[q], epoch);
+    xstore64(rec + R::SIDX, static_cast<unsigned int>(idx), epoch);
+}
+
+// One thread writes a row record and keeps an LDS copy of its payload.
+template <typename T>
+__device__ inline void publish_row(unsigned long long* rec, unsigned int* lds, unsigned int epoch,
+                                   const T (&row)[kLeafW], int tag)
+{
+    using R = Rec<T>;
+    unsigned int w[R::PW];
 #pragma unroll
     for (int c = 0; c < kLeafW; ++c) {
         to_words(row[c], w);
 #pragma unroll
-        for (int q = 0; q < R::PW; ++q) xstore64(rec + R::ROW + R::PW * c + q, w[q], epoch);
+        for (int q = 0; q < R::PW; ++q) {
+            xstore64(rec + R::PW * c + q, w[q], epoch);
+            lds[R::PW * c + q] = w[q];
+        }
     }
-    xstore64(rec + R::TAG, static_cast<unsigned int>(tag), epoch);
+    xstore64(rec + R::RTAG, static_cast<unsigned int>(tag), epoch);
+    lds[R::RTAG] = static_cast<unsigned int>(tag);
 }
 
 // Shared state of one leaf workgroup.
@@ -196,16 +220,64 @@ template <typename T>
 struct LeafShared {
     T wv[4];
     int wi[4];
-    unsigned int rec[kMaxG + 1][Rec<T>::RW];  // payloads of every record of the column
+    unsigned int s[kMaxG][Rec<T>::SW];  // small records of the column
+    unsigned int prow[Rec<T>::RW];      // pivot row (polled)
+    unsigned int mine[Rec<T>::RW];      // this workgroup's candidate row
+    unsigned int jrow[Rec<T>::RW];      // row j (workgroup 0: own copy; pivot owner: polled)
     int p;
-    int abort;
+    int gp;
+    int none;                           // no finite candidate in the column
+    int piv[kLeafW];                    // local pivot row of each column (workgroup 0)
+    T lb[kLeafW][kLeafW + 1];           // epilogue: the unit-lower diagonal block
 };
 
-// Column J of the leaf (J is a compile-time constant so the register rows
-// x[r][J] stay in registers).  Returns false when the bounded wait expired.
-template <typename T, int RPT, int J>
-__device__ __attribute__((always_inline)) inline bool leaf_column(
-    const LeafArgs<T>& a, LeafShared<T>& sh, T (&x)[RPT][kLeafW], int (&tag)[RPT], int base)
+// Polls `cnt` words at `src` (thread e reads word e, e < cnt <= 256 * NP)
+// into `dst` until every word carries `epoch`.  Returns false when the
+// bounded wait expired (uniform across the workgroup).
+template <int NP>
+__device__ inline bool poll_words(const unsigned long long* src, int cnt, unsigned int* dst,
+                                  unsigned int epoch, unsigned long long limit)
+{
+    const int tid = threadIdx.x;
+    unsigned long long spins = 0;
+    while (true) {
+        unsigned long long v[NP];
+#pragma unroll
+        for (int q = 0; q < NP; ++q) {
+            const int e = tid + q * 256;
+            v[q] = (e < cnt) ? xload64(src + e) : 0ULL;
+        }
+        int miss = 0;
+#pragma unroll
+        for (int q = 0; q < NP; ++q) {
+            const int e = tid + q * 256;
+            if (e < cnt) {
+                if (static_cast<unsigned int>(v[q] >> 32) != epoch) miss = 1;
+                else dst[e] = static_cast<unsigned int>(v[q]);
+            }
+        }
+        if (!__syncthreads_or(miss)) return true;
+        if (++spins > limit) return false;
+        __builtin_amdgcn_s_sleep(1);
+    }
+}
+
+// Register rotation: column j of the leaf always sits at index 0 of the
+// register rows, so the column loop is a runtime loop over fixed indices.
+template <typename T, int RPT>
+__device__ inline void rotate_rows(T (&x)[RPT][kLeafW])
+{
+#pragma unroll
+    for (int r = 0; r < RPT; ++r) {
+        const T t0 = x[r][0];
+#pragma unroll
+        for (int c = 0; c < kLeafW - 1; ++c) x[r][c] = x[r][c + 1];
+        x[r][kLeafW - 1] = t0;
+    }
+}
+
+template <typename T, int RPT>
+__global__ __launch_bounds__(256) void getf2_leaf_kernel(LeafArgs<T> a)
 {
     using R = Rec<T>;
     const int tid = threadIdx.x;
@@ -213,177 +285,222 @@ __device__ __attribute__((always_inline)) inline bool leaf_column(
     const int G = a.G;
     const int m = a.m;
     const int w = a.w;
-    constexpr int j = J;
-    const unsigned int epoch = a.epoch0 + static_cast<unsigned int>(j);
-    unsigned long long* slots =
-        a.xrec + static_cast<size_t>(j & 1) * (kMaxG + 1) * R::RW;
-
-    // 1. local candidate: max |a(i,j)| over own rows i >= j, smallest row on ties
-    T best = T(-1);
-    int bi = 0x7fffffff;
-#pragma unroll
-    for (int r = 0; r < RPT; ++r) {
-        const int lrow = base + r * 256 + tid;
-        if (lrow >= j && lrow < m) amax_pick(best, bi, fabs(x[r][j]), lrow);
-    }
-    wave_argmax(best, bi);
-    if ((tid & 63) == 0) {
-        sh.wv[tid >> 6] = best;
-        sh.wi[tid >> 6] = bi;
-    }
-    __syncthreads();
-    T wb = sh.wv[0];
-    int wi = sh.wi[0];
-#pragma unroll
-    for (int q = 1; q < 4; ++q) amax_pick(wb, wi, sh.wv[q], sh.wi[q]);
-
-    // 2. publish the candidate (and row j from workgroup 0)
-    if (wi == 0x7fffffff) {
-        if (tid == 0) {
-            T zero[kLeafW];
-#pragma unroll
-            for (int c = 0; c < kLeafW; ++c) zero[c] = T(0);
-            publish<T>(slots + static_cast<size_t>(g) * R::RW, epoch, 0x7fffffff, T(-1), zero, -1);
-        }
-    } else {
-#pragma unroll
-        for (int r = 0; r < RPT; ++r) {
-            const int lrow = base + r * 256 + tid;
-            if (lrow == wi) publish<T>(slots + static_cast<size_t>(g) * R::RW, epoch, wi, wb, x[r], tag[r]);
-        }
-    }
-    if (g == 0 && tid == j) publish<T>(slots + static_cast<size_t>(G) * R::RW, epoch, j, T(0), x[0], tag[0]);
-
-    // 3. gather every record of this column into LDS (poll until all carry the epoch)
-    const int total = (G + 1) * R::RW;
-    constexpr int kPoll = ((kMaxG + 1) * R::RW + 255) / 256;
-    unsigned long long spins = 0;
-    while (true) {
-        // issue every load of this thread before consuming any
-        unsigned long long v[kPoll];
-#pragma unroll
-        for (int q = 0; q < kPoll; ++q) {
-            const int e = tid + q * 256;
-            v[q] = (e < total) ? xload64(slots + e) : 0ULL;
-        }
-        int miss = 0;
-#pragma unroll
-        for (int q = 0; q < kPoll; ++q) {
-            const int e = tid + q * 256;
-            if (e < total) {
-                if (static_cast<unsigned int>(v[q] >> 32) != epoch) {
-                    miss = 1;
-                } else {
-                    (&sh.rec[0][0])[e] = static_cast<unsigned int>(v[q]);
-                }
-            }
-        }
-        if (!__syncthreads_or(miss)) break;
-        if (++spins > (1ULL << 18)) {
-            if (tid == 0) atomicExch(a.status, 1);
-            return false;
-        }
-        __builtin_amdgcn_s_sleep(1);
-    }
-
-    // 4. global pivot: max over the G candidates, smallest row on ties
-    if (tid < 64) {
-        T gb = T(-1);
-        int gi = 0x7fffffff;
-        for (int t = tid; t < G; t += 64)
-            amax_pick(gb, gi, from_words<T>(&sh.rec[t][R::VAL]), static_cast<int>(sh.rec[t][R::IDX]));
-        wave_argmax(gb, gi);
-        if (tid == 0) sh.p = (gi == 0x7fffffff) ? j : gi;
-    }
-    __syncthreads();
-    const int p = sh.p;
-    const int gp = p / (256 * RPT);
-    const unsigned int* prec = sh.rec[gp];
-    const unsigned int* jrec = sh.rec[G];
-    const T piv = from_words<T>(prec + R::ROW + R::PW * j);
-    if (piv != T(0)) {
-        if (p != j) {
-#pragma unroll
-            for (int r = 0; r < RPT; ++r) {
-                const int lrow = base + r * 256 + tid;
-                if (lrow == p) {
-#pragma unroll
-                    for (int c = 0; c < kLeafW; ++c) x[r][c] = from_words<T>(jrec + R::ROW + R::PW * c);
-                    tag[r] = static_cast<int>(jrec[R::TAG]);
-                } else if (lrow == j) {
-#pragma unroll
-                    for (int c = 0; c < kLeafW; ++c) x[r][c] = from_words<T>(prec + R::ROW + R::PW * c);
-                    tag[r] = static_cast<int>(prec[R::TAG]);
-                }
-            }
-        }
-        const bool recip = fabs(piv) >= sfmin<T>();
-        const T rcp = T(1) / piv;
-        T u[kLeafW];
-#pragma unroll
-        for (int c = j + 1; c < kLeafW; ++c) u[c] = from_words<T>(prec + R::ROW + R::PW * c);
-#pragma unroll
-        for (int r = 0; r < RPT; ++r) {
-            const int lrow = base + r * 256 + tid;
-            if (lrow > j && lrow < m) {
-                const T l = recip ? x[r][j] * rcp : x[r][j] / piv;
-                x[r][j] = l;
-#pragma unroll
-                for (int c = j + 1; c < kLeafW; ++c) x[r][c] -= l * u[c];
-            }
-        }
-    } else if (g == 0 && tid == 0) {
-        atomicCAS(a.info, 0, a.c0 + j + 1);
-    }
-    if (g == 0 && tid == 0) a.ipiv[a.c0 + j] = a.c0 + p + 1;
-    (void)w;
-    return true;
-}
-
-template <typename T, int RPT, int... Js>
-__device__ __attribute__((always_inline)) inline bool leaf_columns(
-    const LeafArgs<T>& a, LeafShared<T>& sh, T (&x)[RPT][kLeafW], int (&tag)[RPT], int base,
-    std::integer_sequence<int, Js...>)
-{
-    bool ok = true;
-    ((ok = ok && (Js >= a.w || leaf_column<T, RPT, Js>(a, sh, x, tag, base))), ...);
-    return ok;
-}
-
-template <typename T, int RPT>
-__global__ __launch_bounds__(256) void getf2_leaf_kernel(LeafArgs<T> a)
-{
-    const int tid = threadIdx.x;
-    const int base = blockIdx.x * 256 * RPT;
+    const int base = g * 256 * RPT;
     __shared__ LeafShared<T> sh;
     // A leaf whose predecessor timed out would wait on stale records: every
     // later leaf exits at once (the host reports the timeout).
     if (__hip_atomic_load(a.status, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0) return;
 
+    // Columns never reached (an expired wait) stay unexchanged: identity local
+    // pivots, and ipiv entries that are valid row indices.
+    if (tid < kLeafW) sh.piv[tid] = tid;
+    if (g == 0 && tid < w) a.ipiv[a.c0 + tid] = a.c0 + tid + 1;
     T x[RPT][kLeafW];
     int tag[RPT];
 #pragma unroll
     for (int r = 0; r < RPT; ++r) {
         const int lrow = base + r * 256 + tid;
-        const bool in = lrow < a.m;
+        const bool in = lrow < m;
 #pragma unroll
         for (int c = 0; c < kLeafW; ++c)
-            x[r][c] = (in && c < a.w) ? a.A[off(a.c0 + lrow, a.c0 + c, a.lda)] : T(0);
+            x[r][c] = (in && c < w) ? a.A[off(a.c0 + lrow, a.c0 + c, a.lda)] : T(0);
         tag[r] = in ? a.orig[a.c0 + lrow] : -1;
     }
 
-    if (!leaf_columns<T, RPT>(a, sh, x, tag, base, std::make_integer_sequence<int, kLeafW>{}))
-        return;
+    constexpr int kPollS = (kMaxG * R::SW + 255) / 256;
+    for (int j = 0; j < w; ++j) {
+        const unsigned int epoch = a.epoch0 + static_cast<unsigned int>(j);
+        unsigned long long* slot = a.xrec + static_cast<size_t>(j & 1) * R::PARITY;
+
+        // 1. local candidate: max |a(i, j)| over own rows i >= j, smallest row on ties
+        T best = T(-1);
+        int bi = kIdxNone;
+#pragma unroll
+        for (int r = 0; r < RPT; ++r) {
+            const int lrow = base + r * 256 + tid;
+            if (lrow >= j && lrow < m) amax_pick(best, bi, fabs(x[r][0]), lrow);
+        }
+        wave_argmax(best, bi);
+        if ((tid & 63) == 0) {
+            sh.wv[tid >> 6] = best;
+            sh.wi[tid >> 6] = bi;
+        }
+        __syncthreads();
+        T wb = sh.wv[0];
+        int wi = sh.wi[0];
+#pragma unroll
+        for (int q = 1; q < 4; ++q) amax_pick(wb, wi, sh.wv[q], sh.wi[q]);
+
+        // 2. publish: the candidate row, then its small record; row j from workgroup 0
+        if (wi == kIdxNone) {
+            if (tid == 0) publish_small<T>(slot + R::S_OFF + g * R::SW, epoch, kIdxNone, T(-1));
+        } else {
+#pragma unroll
+            for (int r = 0; r < RPT; ++r) {
+                const int lrow = base + r * 256 + tid;
+                if (lrow == wi) {
+                    publish_row<T>(slot + R::R_OFF + g * R::RW, sh.mine, epoch, x[r], tag[r]);
+                    publish_small<T>(slot + R::S_OFF + g * R::SW, epoch, wi, wb);
+                }
+            }
+        }
+        if (g == 0 && tid == j) publish_row<T>(slot + R::J_OFF, sh.jrow, epoch, x[0], tag[0]);
+
+        // 3. every small record of the column
+        if (!poll_words<kPollS>(slot + R::S_OFF, G * R::SW, &sh.s[0][0], epoch, a.spin_limit)) {
+            if (tid == 0) {
+                atomicExch(a.status, 1);
+                atomicCAS(a.info, 0, -2);
+            }
+            return;
+        }
+
+        // 4. global pivot: max over the G candidates, smallest row on ties
+        if (tid < 64) {
+            T gb = T(-1);
+            int gi = kIdxNone;
+            for (int t = tid; t < G; t += 64)
+                amax_pick(gb, gi, from_words<T>(&sh.s[t][R::SVAL]), static_cast<int>(sh.s[t][R::SIDX]));
+            wave_argmax(gb, gi);
+            if (tid == 0) {
+                const int p = (gi == kIdxNone) ? j : gi;
+                sh.p = p;
+                sh.gp = p / (256 * RPT);
+                sh.none = gi == kIdxNone;
+            }
+        }
+        __syncthreads();
+        const int p = sh.p;
+        const int gp = sh.gp;
+        const bool none = sh.none != 0;
+
+        // 5. the pivot row (own copy when this workgroup owns it) and, for the
+        //    owner of the pivot row, row j
+        const bool need_p = !none && gp != g;
+        const bool need_j = !none && gp == g && p != j && g != 0;
+        if (need_p || need_j) {
+            const unsigned long long* src = slot + (need_p ? R::R_OFF + gp * R::RW : R::J_OFF);
+            unsigned int* dst = need_p ? sh.prow : sh.jrow;
+            if (!poll_words<1>(src, R::RW, dst, epoch, a.spin_limit)) {
+                if (tid == 0) {
+                    atomicExch(a.status, 1);
+                    atomicCAS(a.info, 0, -2);
+                }
+                return;
+            }
+        }
+        const unsigned int* prow = need_p ? sh.prow : sh.mine;
+        const unsigned int* jrow = sh.jrow;
+
+        // 6. swap, scale by the reciprocal of the pivot (|pivot| >= sfmin, else
+        //    divide, as LAPACK xGETF2), rank-1 update of the later columns
+        const T piv = none ? T(0) : from_words<T>(prow);
+        if (piv != T(0)) {
+            if (p != j) {
+#pragma unroll
+                for (int r = 0; r < RPT; ++r) {
+                    const int lrow = base + r * 256 + tid;
+                    if (lrow == p) {
+#pragma unroll
+                        for (int c = 0; c < kLeafW; ++c) x[r][c] = from_words<T>(jrow + R::PW * c);
+                        tag[r] = static_cast<int>(jrow[R::RTAG]);
+                    } else if (lrow == j) {
+#pragma unroll
+                        for (int c = 0; c < kLeafW; ++c) x[r][c] = from_words<T>(prow + R::PW * c);
+                        tag[r] = static_cast<int>(prow[R::RTAG]);
+                    }
+                }
+            }
+            const bool recip = fabs(piv) >= sfmin<T>();
+            const T rcp = T(1) / piv;
+            // rotated index c is column j + c; indices past kLeafW - 1 - j hold
+            // finished columns and take no update
+            T u[kLeafW];
+#pragma unroll
+            for (int c = 1; c < kLeafW; ++c)
+                u[c] = (c < kLeafW - j) ? from_words<T>(prow + R::PW * c) : T(0);
+#pragma unroll
+            for (int r = 0; r < RPT; ++r) {
+                const int lrow = base + r * 256 + tid;
+                if (lrow > j && lrow < m) {
+                    const T l = recip ? x[r][0] * rcp : x[r][0] / piv;
+                    x[r][0] = l;
+#pragma unroll
+                    for (int c = 1; c < kLeafW; ++c) x[r][c] -= l * u[c];
+                }
+            }
+        } else if (g == 0 && tid == 0) {
+            atomicCAS(a.info, 0, a.c0 + j + 1);
+        }
+        if (g == 0 && tid == 0) {
+            a.ipiv[a.c0 + j] = a.c0 + p + 1;
+            sh.piv[j] = p;
+        }
+        rotate_rows<T, RPT>(x);
+    }
+    for (int j = w; j < kLeafW; ++j) rotate_rows<T, RPT>(x);
 
 #pragma unroll
     for (int r = 0; r < RPT; ++r) {
         const int lrow = base + r * 256 + tid;
-        if (lrow < a.m) {
+        if (lrow < m) {
 #pragma unroll
             for (int c = 0; c < kLeafW; ++c)
-                if (c < a.w) a.A[off(a.c0 + lrow, a.c0 + c, a.lda)] = x[r][c];
+                if (c < w) a.A[off(a.c0 + lrow, a.c0 + c, a.lda)] = x[r][c];
             a.orig[a.c0 + lrow] = tag[r];
         }
+    }
+    if (g != 0) return;
+
+    // Epilogue (workgroup 0).
+    // (a) Move list of the leaf's exchanges: candidate t < w is row t, t >= w
+    //     the pivot row of step t - w; its content came from the row found
+    //     by undoing the exchanges in reverse order.
+    __syncthreads();
+    if (a.mv_src != nullptr && tid < 2 * w) {
+        const int dst = tid < w ? tid : sh.piv[tid - w];
+        int pos = dst;
+        for (int k = w - 1; k >= 0; --k) {
+            const int pk = sh.piv[k];
+            if (pos == k) pos = pk;
+            else if (pos == pk) pos = k;
+        }
+        a.mv_src[tid] = (pos != dst) ? a.c0 + pos : -1;
+        a.mv_dst[tid] = a.c0 + dst;
+    }
+    // (b) Inverse of the unit-lower w x w diagonal block (identity rows past w):
+    //     wave v, half h solves L x = e_c for columns c = 8 v + 4 h + q by
+    //     forward substitution, lane (l % 32) holding row l % 32.
+    if (a.inv == nullptr) return;
+    if (tid < kLeafW) {
+#pragma unroll
+        for (int c = 0; c < kLeafW; ++c)
+            sh.lb[tid][c] = (tid < w && c < tid) ? x[0][c] : T(0);
+    }
+    __syncthreads();
+    const int lane = tid & 63;
+    const int i = lane & 31;
+    const int half = lane & 32;
+    T xv[4];
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+        const int c = (tid >> 6) * 8 + (half >> 3) + q;
+        xv[q] = (i == c) ? T(1) : T(0);
+    }
+#pragma unroll
+    for (int k = 0; k < kLeafW; ++k) {
+        const T lik = sh.lb[i][k];
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            const T xk = __shfl(xv[q], half + k);
+            if (i > k) xv[q] -= lik * xk;
+        }
+    }
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+        const int c = (tid >> 6) * 8 + (half >> 3) + q;
+        a.inv[i + 64 * c] = xv[q];
+        if (a.inv_zero != nullptr) a.inv_zero[i + 64 * c] = T(0);
     }
 }
 
@@ -405,8 +522,8 @@ static void launch_leaf(mifact_handle h, LeafArgs<T>& args)
 }
 
 template <typename T>
-void getf2_leaf(mifact_handle h, T* A, long long lda, int n, int c0, int w,
-                int* ipiv, int* orig, int* info)
+void getf2_leaf(mifact_handle h, T* A, long long lda, int n, int c0, int w, int* ipiv,
+                int* orig, int* info, T* inv, T* inv_zero, int* mv_src, int* mv_dst)
 {
     const int m = n - c0;
     if (w <= 0 || m <= 0) return;
@@ -414,6 +531,8 @@ void getf2_leaf(mifact_handle h, T* A, long long lda, int n, int c0, int w,
         report_status(h, mifact_status_invalid_value, "getf2_leaf: w > 32");
         return;
     }
+    static_assert(2 * Rec<double>::PARITY * 8 <= kExchangeBytes / 2,
+                  "leaf exchange records exceed their half of the exchange area");
     Workspace ws = workspace(h);
     LeafArgs<T> args;
     args.A = A;
@@ -428,10 +547,16 @@ void getf2_leaf(mifact_handle h, T* A, long long lda, int n, int c0, int w,
     args.xrec = reinterpret_cast<unsigned long long*>(ws.exchange);
     args.epoch0 = h->leaf_epoch;
     args.status = h->kernel_status;
-    // Rows per thread: keep G (the workgroups that exchange per column) at 64
+    args.spin_limit = h->wait_limit > 0 ? static_cast<unsigned long long>(h->wait_limit)
+                                        : (1ULL << 18);
+    args.inv = inv;
+    args.inv_zero = inv_zero;
+    args.mv_src = mv_src;
+    args.mv_dst = mv_dst;
+    // Rows per thread: keep G (the workgroups that exchange per column) at 32
     // or below where the registers allow.
     int rpt = 1;
-    while (rpt < 4 && ceil_div(m, 256 * rpt) > 64) rpt *= 2;
+    while (rpt < 4 && ceil_div(m, 256 * rpt) > 32) rpt *= 2;
     args.G = ceil_div(m, 256 * rpt);
     if (args.G > kMaxG) {
         report_status(h, mifact_status_invalid_value, "getf2_leaf: panel taller than 131072 rows");
@@ -440,6 +565,51 @@ void getf2_leaf(mifact_handle h, T* A, long long lda, int n, int c0, int w,
     if (rpt == 1) launch_leaf<T, 1>(h, args);
     else if (rpt == 2) launch_leaf<T, 2>(h, args);
     else launch_leaf<T, 4>(h, args);
+}
+
+// Completes the inverse of a 64 x 64 unit-lower diagonal block from the
+// inverses of its two 32 x 32 halves, which the two leaves of the block
+// left on its diagonal:  inv([A 0; B C]) = [inv(A) 0; -inv(C) B inv(A)  inv(C)].
+// B is read from the factors after the block's last row exchanges.
+template <typename T>
+__global__ __launch_bounds__(256) void inv_pair_kernel(const T* __restrict__ L, long long ldl,
+                                                       T* __restrict__ inv)
+{
+    __shared__ T sb[32][33];
+    __shared__ T sa[32][33];
+    __shared__ T sc[32][33];
+    __shared__ T st[32][33];
+    const int tid = threadIdx.x;
+    for (int e = tid; e < 1024; e += 256) {
+        const int i = e & 31, k = e >> 5;
+        sb[i][k] = L[off(32 + i, k, ldl)];
+        sa[i][k] = inv[i + 64 * k];
+        sc[i][k] = inv[32 + i + 64 * (32 + k)];
+    }
+    __syncthreads();
+    for (int e = tid; e < 1024; e += 256) {
+        const int i = e & 31, k = e >> 5;
+        T s = T(0);
+#pragma unroll 8
+        for (int q = 0; q < 32; ++q) s += sb[i][q] * sa[q][k];
+        st[i][k] = s;
+    }
+    __syncthreads();
+    for (int e = tid; e < 1024; e += 256) {
+        const int i = e & 31, k = e >> 5;
+        T s = T(0);
+#pragma unroll 8
+        for (int q = 0; q < 32; ++q) s += sc[i][q] * st[q][k];
+        inv[32 + i + 64 * k] = -s;
+    }
+}
+
+template <typename T>
+void inv_pair(mifact_handle h, const T* L, long long ldl, T* inv)
+{
+    StageScope scope(h, kStageTrsmLeaf, 2.0 * 2 * 32 * 32 * 32, sizeof(T) * 5.0 * 1024);
+    hipLaunchKernelGGL(inv_pair_kernel<T>, dim3(1), dim3(256), 0, h->stream, L, ldl, inv);
+    MIFACT_LAUNCHED(h);
 }
 
 // ------------------------------------------------------------------ perms
@@ -483,12 +653,22 @@ void clear_int(mifact_handle h, int* p)
 }
 
 // Candidate moves: t < K -> row p0 + t; t >= K -> pivot row of step p0+t-K.
+// Kernels that take row indices from ipiv or a move list skip their work
+// when a GETF2 leaf's bounded wait expired (status != 0): that leaf left its
+// pivots and move list unwritten, and the factorization is already failed
+// (info = -2).
+__device__ inline bool leaf_failed(const int* status)
+{
+    return __hip_atomic_load(status, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0;
+}
+
 __global__ void build_moves_kernel(int p0, int K, const int* __restrict__ ipiv,
                                    const int* __restrict__ orig, const int* __restrict__ inv,
-                                   int* __restrict__ src, int* __restrict__ dst)
+                                   int* __restrict__ src, int* __restrict__ dst,
+                                   const int* status)
 {
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= 2 * K) return;
+    if (t >= 2 * K || leaf_failed(status)) return;
     const int pos = (t < K) ? p0 + t : ipiv[p0 + t - K] - 1;
     const int s = inv[orig[pos]];
     src[t] = (s != pos) ? s : -1;
@@ -502,8 +682,10 @@ template <typename T>
 __global__ __launch_bounds__(256) void apply_moves_lds_kernel(T* __restrict__ A, long long lda,
                                                               int c_begin, int c_end, int CT,
                                                               int nm, const int* __restrict__ src,
-                                                              const int* __restrict__ dst)
+                                                              const int* __restrict__ dst,
+                                                              const int* status)
 {
+    if (leaf_failed(status)) return;
     extern __shared__ unsigned char smem_raw[];
     T* vals = reinterpret_cast<T*>(smem_raw);
     const int c0 = c_begin + blockIdx.x * CT;
@@ -551,8 +733,10 @@ __global__ __launch_bounds__(256) void apply_moves_lds_kernel(T* __restrict__ A,
 // tmp, then scatter (two launches per chunk).
 template <typename T>
 __global__ void gather_moves_kernel(const T* __restrict__ A, long long lda, int c0, int nc,
-                                    int nm, const int* __restrict__ src, T* __restrict__ tmp)
+                                    int nm, const int* __restrict__ src, T* __restrict__ tmp,
+                                    const int* status)
 {
+    if (leaf_failed(status)) return;
     const size_t total = static_cast<size_t>(nc) * nm;
     for (size_t i = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < total;
          i += static_cast<size_t>(gridDim.x) * blockDim.x) {
@@ -566,14 +750,57 @@ __global__ void gather_moves_kernel(const T* __restrict__ A, long long lda, int 
 template <typename T>
 __global__ void scatter_moves_kernel(T* __restrict__ A, long long lda, int c0, int nc, int nm,
                                      const int* __restrict__ src, const int* __restrict__ dst,
-                                     const T* __restrict__ tmp)
+                                     const T* __restrict__ tmp, const int* status)
 {
+    if (leaf_failed(status)) return;
     const size_t total = static_cast<size_t>(nc) * nm;
     for (size_t i = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < total;
          i += static_cast<size_t>(gridDim.x) * blockDim.x) {
         const int t = static_cast<int>(i % nm);
         const int c = static_cast<int>(i / nm);
         if (src[t] >= 0) A[off(dst[t], c0 + c, lda)] = tmp[i];
+    }
+}
+
+template <typename T>
+void apply_moves(mifact_handle h, T* A, long long lda, int c_begin, int c_end, int nm,
+                 const int* src, const int* dst, T* tmp, long long tmp_elems)
+{
+    if (c_end <= c_begin || nm <= 0) return;
+    Workspace ws = workspace(h);
+    StageScope scope(h, kStageRowMoves, 0.0,
+                     2.0 * sizeof(T) * static_cast<double>(nm) * (c_end - c_begin) + 8.0 * nm);
+    const int ncols = c_end - c_begin;
+    constexpr size_t kLdsBudget = 64 * 1024;
+    const size_t per_col = static_cast<size_t>(nm) * sizeof(T);
+    if (per_col <= kLdsBudget) {
+        int CT = static_cast<int>(kLdsBudget / per_col);
+        if (CT > 16) CT = 16;
+        const size_t lds = static_cast<size_t>(CT) * per_col;
+        hipLaunchKernelGGL(apply_moves_lds_kernel<T>, dim3(ceil_div(ncols, CT)), dim3(256), lds,
+                           h->stream, A, lda, c_begin, c_end, CT, nm, src, dst,
+                           h->kernel_status);
+        MIFACT_LAUNCHED(h);
+        return;
+    }
+    // Global staging: the caller's buffer when it holds at least one column
+    // of the move list, else the workspace's own (2n x kStageCols doubles, so
+    // at least kStageCols columns of any move list of order <= n).
+    if (tmp == nullptr || tmp_elems < nm) {
+        tmp = static_cast<T*>(ws.stage);
+        tmp_elems = static_cast<long long>(ws.stage_bytes / sizeof(T));
+    }
+    const int chunk = static_cast<int>(tmp_elems / nm);
+    for (int c0 = c_begin; c0 < c_end; c0 += chunk) {
+        const int nc = (c_end - c0) < chunk ? (c_end - c0) : chunk;
+        const size_t total = static_cast<size_t>(nc) * nm;
+        const int blocks = static_cast<int>(total / 256 + 1 < 65535 ? total / 256 + 1 : 65535);
+        hipLaunchKernelGGL(gather_moves_kernel<T>, dim3(blocks), dim3(256), 0, h->stream, A, lda,
+                           c0, nc, nm, src, tmp, h->kernel_status);
+        MIFACT_LAUNCHED(h);
+        hipLaunchKernelGGL(scatter_moves_kernel<T>, dim3(blocks), dim3(256), 0, h->stream, A, lda,
+                           c0, nc, nm, src, dst, tmp, h->kernel_status);
+        MIFACT_LAUNCHED(h);
     }
 }
 
@@ -585,45 +812,27 @@ void apply_row_moves(mifact_handle h, T* A, long long lda, int c_begin, int c_en
     if (c_end <= c_begin || K <= 0) return;
     Workspace ws = workspace(h);
     const int nm = 2 * K;
-    StageScope scope(h, kStageRowMoves, 0.0,
-                     2.0 * sizeof(T) * static_cast<double>(nm) * (c_end - c_begin) + 20.0 * nm);
     int* src = ws.moves;
     int* dst = ws.moves + 2 * static_cast<size_t>(h->ws_n);
-    hipLaunchKernelGGL(build_moves_kernel, dim3(ceil_div(nm, 256)), dim3(256), 0, h->stream,
-                       p0, K, ipiv, orig, inv, src, dst);
-    MIFACT_LAUNCHED(h);
-    const int ncols = c_end - c_begin;
-    constexpr size_t kLdsBudget = 64 * 1024;
-    const size_t per_col = static_cast<size_t>(nm) * sizeof(T);
-    if (per_col <= kLdsBudget) {
-        int CT = static_cast<int>(kLdsBudget / per_col);
-        if (CT > 16) CT = 16;
-        const size_t lds = static_cast<size_t>(CT) * per_col;
-        hipLaunchKernelGGL(apply_moves_lds_kernel<T>, dim3(ceil_div(ncols, CT)), dim3(256), lds,
-                           h->stream, A, lda, c_begin, c_end, CT, nm, src, dst);
-        MIFACT_LAUNCHED(h);
-        return;
-    }
-    if (tmp == nullptr || tmp_elems < nm) {
-        report_status(h, mifact_status_invalid_value, "apply_row_moves: staging buffer too small");
-        return;
-    }
-    const int chunk = static_cast<int>(tmp_elems / nm);
-    for (int c0 = c_begin; c0 < c_end; c0 += chunk) {
-        const int nc = (c_end - c0) < chunk ? (c_end - c0) : chunk;
-        const size_t total = static_cast<size_t>(nc) * nm;
-        const int blocks = static_cast<int>(total / 256 + 1 < 65535 ? total / 256 + 1 : 65535);
-        hipLaunchKernelGGL(gather_moves_kernel<T>, dim3(blocks), dim3(256), 0, h->stream, A, lda,
-                           c0, nc, nm, src, tmp);
-        MIFACT_LAUNCHED(h);
-        hipLaunchKernelGGL(scatter_moves_kernel<T>, dim3(blocks), dim3(256), 0, h->stream, A, lda,
-                           c0, nc, nm, src, dst, tmp);
+    {
+        StageScope scope(h, kStagePerm, 0.0, 20.0 * nm);
+        hipLaunchKernelGGL(build_moves_kernel, dim3(ceil_div(nm, 256)), dim3(256), 0, h->stream,
+                           p0, K, ipiv, orig, inv, src, dst, h->kernel_status);
         MIFACT_LAUNCHED(h);
     }
+    apply_moves<T>(h, A, lda, c_begin, c_end, nm, src, dst, tmp, tmp_elems);
 }
 
-template void getf2_leaf<float>(mifact_handle, float*, long long, int, int, int, int*, int*, int*);
-template void getf2_leaf<double>(mifact_handle, double*, long long, int, int, int, int*, int*, int*);
+template void getf2_leaf<float>(mifact_handle, float*, long long, int, int, int, int*, int*, int*,
+                                float*, float*, int*, int*);
+template void getf2_leaf<double>(mifact_handle, double*, long long, int, int, int, int*, int*,
+                                 int*, double*, double*, int*, int*);
+template void inv_pair<float>(mifact_handle, const float*, long long, float*);
+template void inv_pair<double>(mifact_handle, const double*, long long, double*);
+template void apply_moves<float>(mifact_handle, float*, long long, int, int, int, const int*,
+                                 const int*, float*, long long);
+template void apply_moves<double>(mifact_handle, double*, long long, int, int, int, const int*,
+                                  const int*, double*, long long);
 template void apply_row_moves<float>(mifact_handle, float*, long long, int, int, int, int,
                                      const int*, const int*, const int*, float*, long long);
 template void apply_row_moves<double>(mifact_handle, double*, long long, int, int, int, int,

@@ -187,8 +187,10 @@ void getrf_panel_impl(mifact_handle h, int m, int n, T* A, int lda, int* ipiv, i
     if (!ensure_workspace(h, m)) return;
     Workspace ws = workspace(h);
     if (info != nullptr) clear_int(h, info);
+    clear_int(h, h->kernel_status);
     perm_identity(h, ws.orig, m);
-    getrf_panel_rec<T>(h, A, lda, m, 0, n, ipiv, ws.orig, ws.inv, h->ws_n, info, 1);
+    getrf_panel_rec<T>(h, A, lda, m, 0, n, ipiv, ws.orig, ws.inv, h->ws_n, info, 1,
+                       panel_inv<T>(h));
 }
 
 }  // namespace

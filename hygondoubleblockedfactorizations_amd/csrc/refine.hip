@@ -561,7 +561,8 @@ __global__ __launch_bounds__(256) void trsv_kernel(const float* F, long long ldf
                                                    const float* inv, float* T, long long ldt,
                                                    int nr, int* flags, int epoch,
                                                    unsigned long long* ticket,
-                                                   unsigned long long ticket_base, int* status)
+                                                   unsigned long long ticket_base, int* status,
+                                                   int spin_limit)
 {
     __shared__ float tile[64][65];
     __shared__ float dinv[64][65];   // the diagonal block's inverse, loaded up front
@@ -611,7 +612,7 @@ __global__ __launch_bounds__(256) void trsv_kernel(const float* F, long long ldf
             int spins = 0;
             while (__hip_atomic_load(flags + bj, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT)
                    != epoch) {
-                if (++spins > kTrsvSpin) {
+                if (++spins > spin_limit) {
                     atomicExch(status, 3);
                     break;
                 }
@@ -823,7 +824,9 @@ void trsv(mifact_handle h, TrsvMode mode, int n, int nrhs, const float* F, long 
             constexpr int R = decltype(nr_tag)::value;
             hipLaunchKernelGGL((trsv_kernel<M, R>), dim3(nblk), dim3(256), 0, h->stream, F, ldf,
                                n, inv, Tk, ldt, nr, w.flags, epoch, w.ticket, base,
-                               h->kernel_status);
+                               h->kernel_status,
+                               h->wait_limit > 0 && h->wait_limit < kTrsvSpin
+                                   ? static_cast<int>(h->wait_limit) : kTrsvSpin);
         };
         auto by_nr = [&](auto mode_tag) {
             if (nr == 1) go(mode_tag, std::integral_constant<int, 1>{});

@@ -51,9 +51,20 @@ mifact_status mifact_get_stream(mifact_handle handle, hipStream_t* stream);
 mifact_status mifact_set_error_mode(mifact_handle handle, mifact_error_mode mode);
 
 /* Returns and clears the first error recorded on the handle since the last
- * call (sticky status for mifact_error_return mode).  It also reports a
- * cooperative panel kernel whose bounded grid barrier timed out. */
+ * call (sticky status for mifact_error_return mode).  It also reports, and
+ * clears, the device status word that a cooperative panel kernel or a
+ * blocked triangular solve raises when one of its bounded waits expires
+ * (mifact_status_kernel_timeout; the GETRF drivers then also leave
+ * *info = -2).  Every pivoting driver call clears that word on entry, so the
+ * report covers the calls since the last driver entry.  Synchronizes the
+ * handle's stream. */
 mifact_status mifact_get_last_error(mifact_handle handle);
+
+/* Number of unsuccessful polls after which a bounded device-side wait gives
+ * up (the GETF2 leaf's pivot exchange, the blocked TRSV's dependency flags);
+ * 0 restores each kernel's default.  A diagnostic knob: a small limit makes
+ * the failure path observable without a hang. */
+mifact_status mifact_set_wait_limit(mifact_handle handle, long long polls);
 
 /* Pre-sizes the workspace for order n (otherwise grown on first use; growing
  * synchronizes the device). */

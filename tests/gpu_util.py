@@ -74,3 +74,86 @@ def pivot_tie_audit(LU, ipiv, ipiv_ref, n, tol):
             pos = s_
     mult = abs(float(LU[pos, k])) if pos > k else 0.0
     return mult >= 1.0 - tol, k, mult
+
+
+def spd_gpu(n, seed):
+    """A = G G^T / n + I (G iid N(0,1)) built in FP64 on the GPU and rounded
+    once to FP32: the same family as the oracle's spd_matrix, for orders
+    where a host FP64 product would dominate a test.  Returns the torch
+    tensor t with A = t.T (column-major buffer), exactly symmetric."""
+    import torch
+    g = torch.Generator(device="cuda")
+    g.manual_seed(seed)
+    G = torch.randn(n, n, generator=g, device="cuda", dtype=torch.float64)
+    A = G @ G.T
+    del G
+    A.div_(float(n))
+    A.diagonal().add_(1.0)
+    A = 0.5 * (A + A.T)
+    return A.float().contiguous()
+
+
+def uniform_gpu(n, seed):
+    """iid Uniform(-1, 1) FP32 on the GPU (column-major buffer t, A = t.T)."""
+    import torch
+    g = torch.Generator(device="cuda")
+    g.manual_seed(seed)
+    A = torch.rand(n, n, generator=g, device="cuda", dtype=torch.float32)
+    return A.mul_(2.0).sub_(1.0)
+
+
+def potrf_residual_gpu(A_buf, L_buf, bs=8192):
+    """||A - L L^T||_F / ||A||_F with FP64 products on the GPU; A_buf, L_buf
+    are column-major buffers (torch tensors t with A = t.T).  A property
+    check for orders the host oracle cannot reach in seconds."""
+    import torch
+    n = A_buf.shape[0]
+    num = 0.0
+    den = 0.0
+    for c0 in range(0, n, bs):
+        c1 = min(n, c0 + bs)
+        acc = A_buf[c0:c1, :].T.double()           # A(:, c0:c1)
+        den += float((acc ** 2).sum())
+        Lr = torch.tril(L_buf.T[:, :c1].double())  # L(:, 0:c1)
+        Lc = torch.tril(L_buf.T[c0:c1, :c1].double(), diagonal=c0)
+        acc -= Lr @ Lc.T
+        num += float((acc ** 2).sum())
+        del acc, Lr, Lc
+    return float(np.sqrt(num / den))
+
+
+def getrf_residual_gpu(A_buf, LU_buf, ipiv, bs=8192):
+    """||P A - L U||_F / ||A||_F with FP64 products on the GPU (blocked over
+    column and k blocks); ipiv 1-based global (numpy)."""
+    import torch
+    n = A_buf.shape[0]
+    perm = np.arange(n)
+    piv = np.asarray(ipiv)[:n] - 1
+    for k in range(n):
+        p = piv[k]
+        if p != k:
+            perm[k], perm[p] = perm[p], perm[k]
+    permd = torch.from_numpy(perm).cuda()
+    X = LU_buf.T
+    M = A_buf.T
+    num = 0.0
+    den = 0.0
+    for c0 in range(0, n, bs):
+        c1 = min(n, c0 + bs)
+        acc = M[permd, c0:c1].double()
+        den += float((M[:, c0:c1].double() ** 2).sum())
+        for k0 in range(0, c1, bs):
+            k1 = min(c1, k0 + bs)
+            Lk = X[:, k0:k1].double()
+            rows = torch.arange(n, device=Lk.device).unsqueeze(1)
+            cols = torch.arange(k0, k1, device=Lk.device).unsqueeze(0)
+            Lk = torch.where(rows > cols, Lk, torch.where(rows == cols, 1.0, 0.0))
+            Uk = X[k0:k1, c0:c1].double()
+            ur = torch.arange(k0, k1, device=Uk.device).unsqueeze(1)
+            uc = torch.arange(c0, c1, device=Uk.device).unsqueeze(0)
+            Uk = torch.where(ur <= uc, Uk, 0.0)
+            acc -= Lk @ Uk
+            del Lk, Uk
+        num += float((acc ** 2).sum())
+        del acc
+    return float(np.sqrt(num / den))
