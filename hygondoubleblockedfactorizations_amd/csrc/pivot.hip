@@ -33,7 +33,7 @@
 namespace mifact {
 
 constexpr int kLeafW = 32;
-constexpr int kMaxG = 128;           // workgroups of a leaf (rows <= 128 * 256 * 4)
+constexpr int kMaxG = 128;           // workgroups of a leaf (rows <= 128 * 512 * 4)
 constexpr int kIdxNone = 0x7fffffff; // "no candidate" row index
 
 template <typename T> __device__ inline T sfmin();
@@ -104,32 +104,79 @@ __device__ inline void wave_argmax(T& v, int& i)
     i = __builtin_amdgcn_readlane(i, 63);
 }
 
+// Wavefront reductions of the device library (declared by the HIP headers
+// only under HIP_ENABLE_EXTRA_WARP_SYNC_TYPES).
+extern "C" __device__ __attribute__((const)) unsigned long long __ockl_wfred_max_u64(
+    unsigned long long);
+extern "C" __device__ __attribute__((const)) double __ockl_wfred_max_f64(double);
+
+// Single-reduction wave arg-max for the leaf (max |v|, smallest label on
+// ties; no candidate: v < 0 or label kIdxNone).  FP32 packs (|v| bits,
+// complement of the label) into one 64-bit key -- non-negative floats order
+// like their bit patterns -- so one wavefront max reduction decides both;
+// FP64 takes the max value, then the smallest label among the lanes holding
+// it.  Returns the wave-uniform result.
+__device__ inline unsigned long long cand_key(float v, int i)
+{
+    return (i == kIdxNone || !(v >= 0.0f))
+               ? 0ULL
+               : (static_cast<unsigned long long>(__float_as_uint(v)) << 32)
+                     | static_cast<unsigned int>(~i);
+}
+
+__device__ inline void key_to_cand(unsigned long long k, float& v, int& i)
+{
+    if (k == 0ULL) {
+        v = -1.0f;
+        i = kIdxNone;
+    } else {
+        v = __uint_as_float(static_cast<unsigned int>(k >> 32));
+        i = ~static_cast<int>(static_cast<unsigned int>(k));
+    }
+}
+
+__device__ inline void wave_argmax_fast(float& v, int& i)
+{
+    key_to_cand(__ockl_wfred_max_u64(cand_key(v, i)), v, i);
+}
+
+__device__ inline void wave_argmax_fast(double& v, int& i)
+{
+    const bool ok = i != kIdxNone && v >= 0.0;
+    const double mx = __ockl_wfred_max_f64(ok ? v : -1.0);
+    const int mi = __ockl_wfred_min_i32((ok && v == mx) ? i : kIdxNone);
+    if (mx >= 0.0) {
+        v = mx;
+        i = mi;
+    } else {
+        v = -1.0;
+        i = kIdxNone;
+    }
+}
+
 // Exchange records of one leaf column, double-buffered by column parity.
 // Every word is 64 bits {payload: 32 bits, epoch: 32 bits} written by one
 // coherent 64-bit store, so a reader that sees the column's epoch in a word
-// also sees that word's payload; readers poll the words they need until all
-// carry the epoch (no counter, no fence).  Per parity:
-//   S[g]  small record of workgroup g: its candidate row and |value|;
-//   R[g]  the candidate row itself (kLeafW values) and its origin tag;
-//   J     row j (written by workgroup 0, read only by the pivot's owner).
-// Every workgroup polls the G small records, picks the pivot, then reads one
-// row record: G * SW + RW words per column instead of every candidate row.
-// A workgroup can only overwrite a parity slot (column j + 2) after it has
-// read every small record of column j + 1, which each workgroup publishes
-// only after it has finished reading column j.
+// also sees that word's payload; readers poll until every word carries the
+// epoch (no counter, no fence).  Workgroup g publishes record g: its pivot
+// candidate (row, |value|), the candidate's whole row and its origin tag.
+// Every workgroup gathers all G records in one round trip, so the pivot row
+// is at hand as soon as the pivot is known.  A workgroup can only overwrite
+// a parity slot (column j + 2) after it has gathered every record of column
+// j + 1, which each workgroup publishes only after it has finished with
+// column j.
 template <typename T>
 struct Rec {
-    static constexpr int PW = sizeof(T) / 4;       // payload words per T
-    static constexpr int SIDX = 0;
-    static constexpr int SVAL = 1;
-    static constexpr int SW = 1 + PW;
-    static constexpr int RTAG = PW * kLeafW;
-    static constexpr int RW = RTAG + 1;
-    static constexpr int S_OFF = 0;
-    static constexpr int R_OFF = kMaxG * SW;
-    static constexpr int J_OFF = R_OFF + kMaxG * RW;
-    static constexpr int PARITY = J_OFF + RW;
+    static constexpr int PW = sizeof(T) / 4;   // payload words per T
+    static constexpr int ROW = 0;               // kLeafW values
+    static constexpr int TAG = PW * kLeafW;     // the row's origin tag
+    static constexpr int VAL = TAG + 1;         // |value| of the candidate
+    static constexpr int IDX = VAL + PW;        // candidate row (label)
+    static constexpr int RW = IDX + 1;          // words per record
+    static constexpr int PARITY = kMaxG * RW;
 };
+
+constexpr int kLeafThreads = 512;    // 8 waves: rows of the panel in registers
 
 template <typename T>
 struct LeafArgs {
@@ -146,7 +193,8 @@ struct LeafArgs {
     unsigned int epoch0;            // epoch of column j is epoch0 + j
     int* status;
     unsigned long long spin_limit;  // unsuccessful polls before giving up
-    int G;
+    int G;                          // workgroups that hold rows
+    int stride;                     // launched blocks per holding workgroup
     T* inv;                         // inverse of the leaf's unit-lower block (ld 64) or null
     T* inv_zero;                    // 32 x 32 block (ld 64) to clear, or null
     int* mv_src;                    // move list of the leaf's pivots (2w entries) or null
@@ -183,109 +231,239 @@ template <> __device__ inline double from_words<double>(const unsigned int* w)
     return __longlong_as_double(static_cast<long long>(b));
 }
 
-// One thread writes a small record.
-template <typename T>
-__device__ inline void publish_small(unsigned long long* rec, unsigned int epoch, int idx, T val)
-{
-    using R = Rec<T>;
-    unsigned int w[R::PW];
-    to_words(val, w);
-#pragma unroll
-    for (int q = 0; q < R::PW; ++q) xstore64(rec + R::SVAL + q, w[q], epoch);
-    xstore64(rec + R::SIDX, static_cast<unsigned int>(idx), epoch);
-}
-
-// One thread writes a row record and keeps an LDS copy of its payload.
-template <typename T>
-__device__ inline void publish_row(unsigned long long* rec, unsigned int* lds, unsigned int epoch,
-                                   const T (&row)[kLeafW], int tag)
-{
-    using R = Rec<T>;
-    unsigned int w[R::PW];
-#pragma unroll
-    for (int c = 0; c < kLeafW; ++c) {
-        to_words(row[c], w);
-#pragma unroll
-        for (int q = 0; q < R::PW; ++q) {
-            xstore64(rec + R::PW * c + q, w[q], epoch);
-            lds[R::PW * c + q] = w[q];
-        }
-    }
-    xstore64(rec + R::RTAG, static_cast<unsigned int>(tag), epoch);
-    lds[R::RTAG] = static_cast<unsigned int>(tag);
-}
-
 // Shared state of one leaf workgroup.
 template <typename T>
 struct LeafShared {
-    T wv[4];
-    int wi[4];
-    unsigned int s[kMaxG][Rec<T>::SW];  // small records of the column
-    unsigned int prow[Rec<T>::RW];      // pivot row (polled)
-    unsigned int mine[Rec<T>::RW];      // this workgroup's candidate row
-    unsigned int jrow[Rec<T>::RW];      // row j (workgroup 0: own copy; pivot owner: polled)
-    int p;
-    int gp;
-    int none;                           // no finite candidate in the column
-    int piv[kLeafW];                    // local pivot row of each column (workgroup 0)
-    T lb[kLeafW][kLeafW + 1];           // epilogue: the unit-lower diagonal block
+    T wv[kLeafThreads / 64];
+    int wi[kLeafThreads / 64];
+    unsigned int stage[kLeafThreads / 64][Rec<T>::RW];  // each wave's candidate row
+    unsigned int rec[kMaxG][Rec<T>::RW];  // payloads of every record of the column
+    int abort;                            // a bounded wait expired
+    int piv[kLeafW];                      // local pivot row of each column (workgroup 0)
+    T lb[kLeafW][kLeafW + 1];             // workgroup 0: the leaf's first kLeafW rows
 };
 
-// Polls `cnt` words at `src` (thread e reads word e, e < cnt <= 256 * NP)
-// into `dst` until every word carries `epoch`.  Returns false when the
-// bounded wait expired (uniform across the workgroup).
-template <int NP>
-__device__ inline bool poll_words(const unsigned long long* src, int cnt, unsigned int* dst,
-                                  unsigned int epoch, unsigned long long limit)
-{
-    const int tid = threadIdx.x;
-    unsigned long long spins = 0;
-    while (true) {
-        unsigned long long v[NP];
-#pragma unroll
-        for (int q = 0; q < NP; ++q) {
-            const int e = tid + q * 256;
-            v[q] = (e < cnt) ? xload64(src + e) : 0ULL;
-        }
-        int miss = 0;
-#pragma unroll
-        for (int q = 0; q < NP; ++q) {
-            const int e = tid + q * 256;
-            if (e < cnt) {
-                if (static_cast<unsigned int>(v[q] >> 32) != epoch) miss = 1;
-                else dst[e] = static_cast<unsigned int>(v[q]);
-            }
-        }
-        if (!__syncthreads_or(miss)) return true;
-        if (++spins > limit) return false;
-        __builtin_amdgcn_s_sleep(1);
-    }
-}
+// Register rotation: the leaf's columns are processed in groups of kLeafU
+// with register indices 0 .. kLeafU - 1, then every row rotates left by
+// kLeafU, so the column loop is a runtime loop over fixed register indices.
+// (Groups of 2 halve the rotation moves; 4 would exceed the register budget
+// of 4 rows per thread.)
+constexpr int kLeafU = 2;
 
-// Register rotation: column j of the leaf always sits at index 0 of the
-// register rows, so the column loop is a runtime loop over fixed indices.
 template <typename T, int RPT>
 __device__ inline void rotate_rows(T (&x)[RPT][kLeafW])
 {
 #pragma unroll
     for (int r = 0; r < RPT; ++r) {
-        const T t0 = x[r][0];
+        T t[kLeafU];
 #pragma unroll
-        for (int c = 0; c < kLeafW - 1; ++c) x[r][c] = x[r][c + 1];
-        x[r][kLeafW - 1] = t0;
+        for (int u = 0; u < kLeafU; ++u) t[u] = x[r][u];
+#pragma unroll
+        for (int c = 0; c < kLeafW - kLeafU; ++c) x[r][c] = x[r][c + kLeafU];
+#pragma unroll
+        for (int u = 0; u < kLeafU; ++u) x[r][kLeafW - kLeafU + u] = t[u];
     }
 }
 
-template <typename T, int RPT>
-__global__ __launch_bounds__(256) void getf2_leaf_kernel(LeafArgs<T> a)
+// Column j = j0 + U of a leaf (register index U); see getf2_leaf_kernel.
+// Returns false when a bounded wait expired (uniform over the workgroup).
+template <typename T, int RPT, int NP, int U>
+__device__ __attribute__((always_inline)) inline bool leaf_column(
+    const LeafArgs<T>& a, LeafShared<T>& sh, T (&x)[RPT][kLeafW], int (&tag)[RPT],
+    int (&lab)[RPT], int j0, int g)
 {
     using R = Rec<T>;
+    constexpr int NW = kLeafThreads / 64;
+    const int j = j0 + U;
     const int tid = threadIdx.x;
-    const int g = blockIdx.x;
     const int G = a.G;
+    const unsigned int epoch = a.epoch0 + static_cast<unsigned int>(j);
+    unsigned long long* slot = a.xrec + static_cast<size_t>(j & 1) * R::PARITY;
+
+    // 1. local candidate; each wave's winning lane stages its row in LDS
+    T best = T(-1);
+    int bi = kIdxNone;
+#pragma unroll
+    for (int r = 0; r < RPT; ++r)
+        if (lab[r] >= j && lab[r] != kIdxNone) amax_pick(best, bi, fabs(x[r][U]), lab[r]);
+    wave_argmax_fast(best, bi);
+    const int wave = tid >> 6;
+#pragma unroll
+    for (int r = 0; r < RPT; ++r) {
+        if (bi != kIdxNone && lab[r] == bi) {
+            unsigned int* st = sh.stage[wave];
+#pragma unroll
+            for (int c = 0; c < kLeafW; ++c) to_words(x[r][c], st + R::PW * c);
+            st[R::TAG] = static_cast<unsigned int>(tag[r]);
+        }
+    }
+    if ((tid & 63) == 0) {
+        sh.wv[wave] = best;
+        sh.wi[wave] = bi;
+    }
+    __syncthreads();
+    T wb = sh.wv[0];
+    int wi = sh.wi[0];
+    int qw = 0;
+#pragma unroll
+    for (int q = 1; q < NW; ++q) {
+        const T ov = sh.wv[q];
+        const int oi = sh.wi[q];
+        if (ov > wb || (ov == wb && oi < wi)) {
+            wb = ov;
+            wi = oi;
+            qw = q;
+        }
+    }
+
+    // 2. publish the record, one word per thread
+    if (tid < R::RW) {
+        unsigned int word;
+        if (tid < R::VAL) {
+            word = (wi == kIdxNone) ? (tid == R::TAG ? 0xffffffffu : 0u) : sh.stage[qw][tid];
+        } else if (tid < R::IDX) {
+            unsigned int vw[R::PW];
+            to_words(wi == kIdxNone ? T(-1) : wb, vw);
+            word = vw[tid - R::VAL];
+        } else {
+            word = static_cast<unsigned int>(wi);
+        }
+        xstore64(slot + static_cast<size_t>(g) * R::RW + tid, word, epoch);
+    }
+
+    // 3. gather: each thread polls its own words of the G records until all
+    //    carry the column's epoch; one barrier joins the workgroup
+    {
+        const int total = G * R::RW;
+        unsigned int pending = 0;
+#pragma unroll
+        for (int q = 0; q < NP; ++q)
+            if (tid + q * kLeafThreads < total) pending |= 1u << q;
+        unsigned long long spins = 0;
+        while (pending != 0) {
+            unsigned long long v[NP];
+#pragma unroll
+            for (int q = 0; q < NP; ++q)
+                v[q] = ((pending >> q) & 1u) ? xload64(slot + tid + q * kLeafThreads) : 0ULL;
+#pragma unroll
+            for (int q = 0; q < NP; ++q) {
+                if (((pending >> q) & 1u) && static_cast<unsigned int>(v[q] >> 32) == epoch) {
+                    (&sh.rec[0][0])[tid + q * kLeafThreads] = static_cast<unsigned int>(v[q]);
+                    pending &= ~(1u << q);
+                }
+            }
+            if (pending == 0) break;
+            if (++spins > a.spin_limit) {
+                sh.abort = 1;
+                break;
+            }
+            __builtin_amdgcn_s_sleep(1);
+        }
+    }
+    __syncthreads();
+    if (sh.abort) {
+        if (tid == 0) {
+            atomicExch(a.status, 1);
+            atomicCAS(a.info, 0, -2);
+        }
+        return false;
+    }
+
+    // 4. global pivot, computed by every wave from the gathered records (lane
+    //    t takes record t, then the wave arg-max; no barrier): max over the G
+    //    candidates, smallest label on ties (the key carries the record index
+    //    in its low 8 bits)
+    T gb = T(-1);
+    int gk = kIdxNone;
+    for (int t = tid & 63; t < G; t += 64) {
+        const int idx = static_cast<int>(sh.rec[t][R::IDX]);
+        amax_pick(gb, gk, from_words<T>(&sh.rec[t][R::VAL]),
+                  idx == kIdxNone ? kIdxNone : (idx << 8) | t);
+    }
+    wave_argmax_fast(gb, gk);
+    const bool none = gk == kIdxNone;
+    const int p = none ? j : (gk >> 8);
+    const unsigned int* prow = sh.rec[none ? 0 : (gk & 0xff)];
+    const T piv = none ? T(0) : from_words<T>(prow + R::ROW + R::PW * U);
+    if (piv != T(0)) {
+        const bool recip = fabs(piv) >= sfmin<T>();
+        const T rcp = T(1) / piv;
+        T l[RPT];
+#pragma unroll
+        for (int r = 0; r < RPT; ++r) {
+            if (lab[r] == p) lab[r] = j;
+            else if (lab[r] == j) lab[r] = p;
+        }
+        if (recip) {
+#pragma unroll
+            for (int r = 0; r < RPT; ++r) l[r] = x[r][U] * rcp;
+        } else {
+#pragma unroll
+            for (int r = 0; r < RPT; ++r) l[r] = x[r][U] / piv;
+        }
+#pragma unroll
+        for (int r = 0; r < RPT; ++r) {
+            const bool upd = lab[r] > j && lab[r] != kIdxNone;
+            l[r] = upd ? l[r] : T(0);
+            if (upd) x[r][U] = l[r];
+        }
+        // register index c holds column j0 + c; indices from kLeafW - j0 on
+        // hold finished columns (wrapped) and take no update (every load is
+        // issued, the finished ones are then zeroed)
+        T u[kLeafW];
+#pragma unroll
+        for (int c = U + 1; c < kLeafW; ++c) u[c] = from_words<T>(prow + R::ROW + R::PW * c);
+#pragma unroll
+        for (int c = U + 1; c < kLeafW; ++c) {
+            const T uc = (c < kLeafW - j0) ? u[c] : T(0);
+#pragma unroll
+            for (int r = 0; r < RPT; ++r) x[r][c] -= l[r] * uc;
+        }
+    } else if (g == 0 && tid == 0) {
+        atomicCAS(a.info, 0, a.c0 + j + 1);
+    }
+    // workgroup 0 keeps row j of the factors (final now: its multipliers are
+    // set and it takes no further update), in column order
+    if (g == 0 && tid < kLeafW) {
+        const int rc = (tid - j0 + kLeafW) % kLeafW;
+        sh.lb[j][tid] = none ? T(0) : from_words<T>(prow + R::ROW + R::PW * rc);
+    }
+    if (g == 0 && tid == 0) {
+        a.ipiv[a.c0 + j] = a.c0 + p + 1;
+        sh.piv[j] = p;
+    }
+    return true;
+}
+
+// Cooperative partial-pivot LU of an m x w leaf panel (w <= 32).  Blocks
+// b with b % stride == 0 hold rows: workgroup g = b / stride loads rows
+// [g * 512 * RPT, (g + 1) * 512 * RPT) of the panel, RPT per thread, into
+// registers for the whole leaf; the other blocks exit at once.  The
+// hardware deals consecutive blocks to the XCDs in turn, so a stride of 8
+// keeps the exchanging workgroups on one XCD.  Rows never move between
+// threads: each register row carries a label, its current row index in the
+// LAPACK sense, and a row exchange swaps two labels.  At the end every row
+// is stored at its label.  Per column j:
+//   1. local candidate: wave64 DPP arg-max, then across the 8 waves in LDS
+//      (max |a(i, j)| over labels i >= j, smallest label on ties: IxAMAX);
+//   2. publish the record;
+//   3. gather every record of the column (one round trip once all are out);
+//   4. the global pivot p (every thread reduces the G records itself),
+//      relabel j <-> p, scale by the reciprocal of the pivot (|pivot| >=
+//      sfmin, else divide, as LAPACK xGETF2) and apply the rank-1 update to
+//      own rows.
+// Workgroup 0 keeps each column's pivot row: they are the leaf's first rows,
+// used by the epilogue (inverse of the unit-lower block).
+template <typename T, int RPT, int NP>
+__global__ __launch_bounds__(kLeafThreads) void getf2_leaf_kernel(LeafArgs<T> a)
+{
+    constexpr int ROWS = kLeafThreads * RPT;
+    if (static_cast<int>(blockIdx.x) % a.stride != 0) return;
+    const int tid = threadIdx.x;
+    const int g = static_cast<int>(blockIdx.x) / a.stride;
     const int m = a.m;
     const int w = a.w;
-    const int base = g * 256 * RPT;
+    const int base = g * ROWS;
     __shared__ LeafShared<T> sh;
     // A leaf whose predecessor timed out would wait on stale records: every
     // later leaf exits at once (the host reports the timeout).
@@ -294,160 +472,39 @@ __global__ __launch_bounds__(256) void getf2_leaf_kernel(LeafArgs<T> a)
     // Columns never reached (an expired wait) stay unexchanged: identity local
     // pivots, and ipiv entries that are valid row indices.
     if (tid < kLeafW) sh.piv[tid] = tid;
+    if (tid == 0) sh.abort = 0;
     if (g == 0 && tid < w) a.ipiv[a.c0 + tid] = a.c0 + tid + 1;
     T x[RPT][kLeafW];
     int tag[RPT];
+    int lab[RPT];
 #pragma unroll
     for (int r = 0; r < RPT; ++r) {
-        const int lrow = base + r * 256 + tid;
+        const int lrow = base + r * kLeafThreads + tid;
         const bool in = lrow < m;
 #pragma unroll
         for (int c = 0; c < kLeafW; ++c)
             x[r][c] = (in && c < w) ? a.A[off(a.c0 + lrow, a.c0 + c, a.lda)] : T(0);
         tag[r] = in ? a.orig[a.c0 + lrow] : -1;
+        lab[r] = in ? lrow : kIdxNone;
     }
 
-    constexpr int kPollS = (kMaxG * R::SW + 255) / 256;
-    for (int j = 0; j < w; ++j) {
-        const unsigned int epoch = a.epoch0 + static_cast<unsigned int>(j);
-        unsigned long long* slot = a.xrec + static_cast<size_t>(j & 1) * R::PARITY;
-
-        // 1. local candidate: max |a(i, j)| over own rows i >= j, smallest row on ties
-        T best = T(-1);
-        int bi = kIdxNone;
-#pragma unroll
-        for (int r = 0; r < RPT; ++r) {
-            const int lrow = base + r * 256 + tid;
-            if (lrow >= j && lrow < m) amax_pick(best, bi, fabs(x[r][0]), lrow);
-        }
-        wave_argmax(best, bi);
-        if ((tid & 63) == 0) {
-            sh.wv[tid >> 6] = best;
-            sh.wi[tid >> 6] = bi;
-        }
-        __syncthreads();
-        T wb = sh.wv[0];
-        int wi = sh.wi[0];
-#pragma unroll
-        for (int q = 1; q < 4; ++q) amax_pick(wb, wi, sh.wv[q], sh.wi[q]);
-
-        // 2. publish: the candidate row, then its small record; row j from workgroup 0
-        if (wi == kIdxNone) {
-            if (tid == 0) publish_small<T>(slot + R::S_OFF + g * R::SW, epoch, kIdxNone, T(-1));
-        } else {
-#pragma unroll
-            for (int r = 0; r < RPT; ++r) {
-                const int lrow = base + r * 256 + tid;
-                if (lrow == wi) {
-                    publish_row<T>(slot + R::R_OFF + g * R::RW, sh.mine, epoch, x[r], tag[r]);
-                    publish_small<T>(slot + R::S_OFF + g * R::SW, epoch, wi, wb);
-                }
-            }
-        }
-        if (g == 0 && tid == j) publish_row<T>(slot + R::J_OFF, sh.jrow, epoch, x[0], tag[0]);
-
-        // 3. every small record of the column
-        if (!poll_words<kPollS>(slot + R::S_OFF, G * R::SW, &sh.s[0][0], epoch, a.spin_limit)) {
-            if (tid == 0) {
-                atomicExch(a.status, 1);
-                atomicCAS(a.info, 0, -2);
-            }
-            return;
-        }
-
-        // 4. global pivot: max over the G candidates, smallest row on ties
-        if (tid < 64) {
-            T gb = T(-1);
-            int gi = kIdxNone;
-            for (int t = tid; t < G; t += 64)
-                amax_pick(gb, gi, from_words<T>(&sh.s[t][R::SVAL]), static_cast<int>(sh.s[t][R::SIDX]));
-            wave_argmax(gb, gi);
-            if (tid == 0) {
-                const int p = (gi == kIdxNone) ? j : gi;
-                sh.p = p;
-                sh.gp = p / (256 * RPT);
-                sh.none = gi == kIdxNone;
-            }
-        }
-        __syncthreads();
-        const int p = sh.p;
-        const int gp = sh.gp;
-        const bool none = sh.none != 0;
-
-        // 5. the pivot row (own copy when this workgroup owns it) and, for the
-        //    owner of the pivot row, row j
-        const bool need_p = !none && gp != g;
-        const bool need_j = !none && gp == g && p != j && g != 0;
-        if (need_p || need_j) {
-            const unsigned long long* src = slot + (need_p ? R::R_OFF + gp * R::RW : R::J_OFF);
-            unsigned int* dst = need_p ? sh.prow : sh.jrow;
-            if (!poll_words<1>(src, R::RW, dst, epoch, a.spin_limit)) {
-                if (tid == 0) {
-                    atomicExch(a.status, 1);
-                    atomicCAS(a.info, 0, -2);
-                }
-                return;
-            }
-        }
-        const unsigned int* prow = need_p ? sh.prow : sh.mine;
-        const unsigned int* jrow = sh.jrow;
-
-        // 6. swap, scale by the reciprocal of the pivot (|pivot| >= sfmin, else
-        //    divide, as LAPACK xGETF2), rank-1 update of the later columns
-        const T piv = none ? T(0) : from_words<T>(prow);
-        if (piv != T(0)) {
-            if (p != j) {
-#pragma unroll
-                for (int r = 0; r < RPT; ++r) {
-                    const int lrow = base + r * 256 + tid;
-                    if (lrow == p) {
-#pragma unroll
-                        for (int c = 0; c < kLeafW; ++c) x[r][c] = from_words<T>(jrow + R::PW * c);
-                        tag[r] = static_cast<int>(jrow[R::RTAG]);
-                    } else if (lrow == j) {
-#pragma unroll
-                        for (int c = 0; c < kLeafW; ++c) x[r][c] = from_words<T>(prow + R::PW * c);
-                        tag[r] = static_cast<int>(prow[R::RTAG]);
-                    }
-                }
-            }
-            const bool recip = fabs(piv) >= sfmin<T>();
-            const T rcp = T(1) / piv;
-            // rotated index c is column j + c; indices past kLeafW - 1 - j hold
-            // finished columns and take no update
-            T u[kLeafW];
-#pragma unroll
-            for (int c = 1; c < kLeafW; ++c)
-                u[c] = (c < kLeafW - j) ? from_words<T>(prow + R::PW * c) : T(0);
-#pragma unroll
-            for (int r = 0; r < RPT; ++r) {
-                const int lrow = base + r * 256 + tid;
-                if (lrow > j && lrow < m) {
-                    const T l = recip ? x[r][0] * rcp : x[r][0] / piv;
-                    x[r][0] = l;
-#pragma unroll
-                    for (int c = 1; c < kLeafW; ++c) x[r][c] -= l * u[c];
-                }
-            }
-        } else if (g == 0 && tid == 0) {
-            atomicCAS(a.info, 0, a.c0 + j + 1);
-        }
-        if (g == 0 && tid == 0) {
-            a.ipiv[a.c0 + j] = a.c0 + p + 1;
-            sh.piv[j] = p;
-        }
+    
+    int j0 = 0;
+    for (; j0 < w; j0 += kLeafU) {
+        if (!leaf_column<T, RPT, NP, 0>(a, sh, x, tag, lab, j0, g)) return;
+        if (j0 + 1 < w && !leaf_column<T, RPT, NP, 1>(a, sh, x, tag, lab, j0, g)) return;
         rotate_rows<T, RPT>(x);
     }
-    for (int j = w; j < kLeafW; ++j) rotate_rows<T, RPT>(x);
+    for (; j0 < kLeafW; j0 += kLeafU) rotate_rows<T, RPT>(x);
 
 #pragma unroll
     for (int r = 0; r < RPT; ++r) {
-        const int lrow = base + r * 256 + tid;
-        if (lrow < m) {
+        if (lab[r] != kIdxNone) {
+            const int row = a.c0 + lab[r];
 #pragma unroll
             for (int c = 0; c < kLeafW; ++c)
-                if (c < w) a.A[off(a.c0 + lrow, a.c0 + c, a.lda)] = x[r][c];
-            a.orig[a.c0 + lrow] = tag[r];
+                if (c < w) a.A[off(row, a.c0 + c, a.lda)] = x[r][c];
+            a.orig[row] = tag[r];
         }
     }
     if (g != 0) return;
@@ -469,15 +526,9 @@ __global__ __launch_bounds__(256) void getf2_leaf_kernel(LeafArgs<T> a)
         a.mv_dst[tid] = a.c0 + dst;
     }
     // (b) Inverse of the unit-lower w x w diagonal block (identity rows past w):
-    //     wave v, half h solves L x = e_c for columns c = 8 v + 4 h + q by
+    //     wave v < 4, half h solves L x = e_c for columns c = 8 v + 4 h + q by
     //     forward substitution, lane (l % 32) holding row l % 32.
-    if (a.inv == nullptr) return;
-    if (tid < kLeafW) {
-#pragma unroll
-        for (int c = 0; c < kLeafW; ++c)
-            sh.lb[tid][c] = (tid < w && c < tid) ? x[0][c] : T(0);
-    }
-    __syncthreads();
+    if (a.inv == nullptr || tid >= 256) return;
     const int lane = tid & 63;
     const int i = lane & 31;
     const int half = lane & 32;
@@ -489,7 +540,7 @@ __global__ __launch_bounds__(256) void getf2_leaf_kernel(LeafArgs<T> a)
     }
 #pragma unroll
     for (int k = 0; k < kLeafW; ++k) {
-        const T lik = sh.lb[i][k];
+        const T lik = (i < w && k < i) ? sh.lb[i][k] : T(0);
 #pragma unroll
         for (int q = 0; q < 4; ++q) {
             const T xk = __shfl(xv[q], half + k);
@@ -504,7 +555,7 @@ __global__ __launch_bounds__(256) void getf2_leaf_kernel(LeafArgs<T> a)
     }
 }
 
-template <typename T, int RPT>
+template <typename T, int RPT, int NP>
 static void launch_leaf(mifact_handle h, LeafArgs<T>& args)
 {
     const double m = args.m, w = args.w;
@@ -512,13 +563,23 @@ static void launch_leaf(mifact_handle h, LeafArgs<T>& args)
                      sizeof(T) * 2.0 * m * w + 8.0 * m);
     void* params[] = {&args};
     hipError_t e = hipLaunchCooperativeKernel(
-        reinterpret_cast<const void*>(&getf2_leaf_kernel<T, RPT>), dim3(args.G),
-        dim3(256), params, 0, h->stream);
+        reinterpret_cast<const void*>(&getf2_leaf_kernel<T, RPT, NP>), dim3(args.G * args.stride),
+        dim3(kLeafThreads), params, 0, h->stream);
     if (e != hipSuccess) {
         report_hip(h, e, "hipLaunchCooperativeKernel(getf2_leaf)", __FILE__, __LINE__);
         return;
     }
     h->leaf_epoch += static_cast<unsigned int>(args.w);
+}
+
+template <typename T, int RPT>
+static void launch_leaf_np(mifact_handle h, LeafArgs<T>& args)
+{
+    // words gathered per thread: up to 16 records in the common case
+    constexpr int kSmall = (16 * Rec<T>::RW + kLeafThreads - 1) / kLeafThreads;
+    constexpr int kLarge = (kMaxG * Rec<T>::RW + kLeafThreads - 1) / kLeafThreads;
+    if (args.G <= 16) launch_leaf<T, RPT, kSmall>(h, args);
+    else launch_leaf<T, RPT, kLarge>(h, args);
 }
 
 template <typename T>
@@ -553,18 +614,25 @@ void getf2_leaf(mifact_handle h, T* A, long long lda, int n, int c0, int w, int*
     args.inv_zero = inv_zero;
     args.mv_src = mv_src;
     args.mv_dst = mv_dst;
-    // Rows per thread: keep G (the workgroups that exchange per column) at 32
-    // or below where the registers allow.
+    // Rows per thread: grow (FP32 up to 4, FP64 up to 2 x 32 values per
+    // thread in the registers of a 512-thread workgroup) while more than 16
+    // workgroups would have to exchange.
+    constexpr int max_rpt = sizeof(T) == 4 ? 4 : 2;
     int rpt = 1;
-    while (rpt < 4 && ceil_div(m, 256 * rpt) > 32) rpt *= 2;
-    args.G = ceil_div(m, 256 * rpt);
+    while (rpt < max_rpt && ceil_div(m, kLeafThreads * rpt) > 16) rpt *= 2;
+    args.G = ceil_div(m, kLeafThreads * rpt);
     if (args.G > kMaxG) {
-        report_status(h, mifact_status_invalid_value, "getf2_leaf: panel taller than 131072 rows");
+        report_status(h, mifact_status_invalid_value, "getf2_leaf: panel too tall");
         return;
     }
-    if (rpt == 1) launch_leaf<T, 1>(h, args);
-    else if (rpt == 2) launch_leaf<T, 2>(h, args);
-    else launch_leaf<T, 4>(h, args);
+    // One CU per launched block: at most num_cu blocks, stride 8 (one XCD)
+    // where that fits.
+    int stride = 8;
+    while (stride > 1 && args.G * stride > h->num_cu) stride /= 2;
+    args.stride = stride;
+    if (rpt == 1) launch_leaf_np<T, 1>(h, args);
+    else if (rpt == 2) launch_leaf_np<T, 2>(h, args);
+    else if constexpr (max_rpt >= 4) launch_leaf_np<T, 4>(h, args);
 }
 
 // Completes the inverse of a 64 x 64 unit-lower diagonal block from the
