@@ -34,6 +34,7 @@ SIGNATURES = {
     # potrf/double_blocked_potrf.h
     "double_blocked_potrf": [P, P, I, I, I, I, P],
     "double_blocked_potrf_dbtrsm": [P, P, I, I, I, I, P],
+    "double_blocked_potrf_noalloc": [P, P, I, I, I, I, P],
     "double_blocked_potrf_kblocked_syrk": [P, P, I, I, I, I, I, P],
     "double_blocked_potrf_kblocked_syrk_adaptive_gemm": [P, P, I, I, I, I, I, P],
     "double_blocked_potrf_kblocked_syrk_adaptive_gemm_configured":

@@ -164,6 +164,8 @@ constexpr size_t kExchangeBytes = 1 << 20;  // 2 parities x (G+1) records
 constexpr int kStageCols = 16;              // columns per row-move staging chunk (FP64)
 
 bool ensure_workspace(mifact_handle h, int n);
+// Set-up of the allocation-free POTRF entry point on the current device.
+bool noalloc_setup(mifact_handle h);
 Workspace workspace(mifact_handle h);
 
 // Brackets the kernel launches issued in its lifetime with an event pair

@@ -8,9 +8,11 @@
 namespace mifact {
 
 // Double-blocked POTRF (reference schedule); trailing updates in `prec`.
+// inv_scratch: (b + 64) x 64 elements for the diagonal-block inverses, or
+// null to use the handle's workspace.
 template <typename T>
 void double_blocked_potrf_impl(mifact_handle h, T* A, int lda, int n, int nb, int b, int* info,
-                               bool use_dbtrsm, LowPrecision prec);
+                               bool use_dbtrsm, LowPrecision prec, T* inv_scratch = nullptr);
 
 // Double-blocked GETRF (reference schedule); trailing updates in `prec`.
 // On return the workspace's orig[] holds the final row permutation

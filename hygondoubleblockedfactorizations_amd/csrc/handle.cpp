@@ -226,6 +226,11 @@ mifact_status mifact_create_handle(mifact_handle* handle)
     h->barrier = static_cast<unsigned long long*>(bar);
     h->kernel_status = reinterpret_cast<int*>(static_cast<char*>(bar) + 64);
     h->barrier_base = 0;
+    if (!mifact::noalloc_setup(h)) {
+        (void)hipFree(bar);
+        delete h;
+        return mifact_status_hip_error;
+    }
     *handle = h;
     return mifact_status_success;
 }

@@ -6,6 +6,8 @@
 // replaced by the gfx950 kernels through the recursive building blocks.
 #include <hip/hip_runtime.h>
 
+#include <cstdio>
+#include <mutex>
 #include <type_traits>
 
 #include "blas3.hpp"
@@ -72,7 +74,7 @@ void update_nt(mifact_handle h, LowPrecision prec, GemmShape shape, int M, int N
 
 template <typename T>
 void double_blocked_potrf_impl(mifact_handle h, T* A, int lda_i, int n, int nb, int b,
-                               int* info, bool use_dbtrsm, LowPrecision prec)
+                               int* info, bool use_dbtrsm, LowPrecision prec, T* inv_scratch)
 {
     if (h == nullptr) {
         report_status(nullptr, mifact_status_invalid_handle, "double_blocked_potrf: null handle");
@@ -83,8 +85,13 @@ void double_blocked_potrf_impl(mifact_handle h, T* A, int lda_i, int n, int nb, 
     if (b <= 0) b = nb;
     b = imin(b, nb);
     const long long lda = lda_i;
-    if (!ensure_workspace(h, n)) return;
-    T* inv = invdiag<T>(h);
+    // The diagonal blocks' inverses (the only scratch of this schedule):
+    // (b + 64) x 64 elements, from the workspace or from the caller.
+    T* inv = inv_scratch;
+    if (inv == nullptr) {
+        if (!ensure_workspace(h, n)) return;
+        inv = invdiag<T>(h);
+    }
     if (info != nullptr) clear_int(h, info);
 
     for (int outer = 0; outer < n; outer += nb) {
@@ -133,9 +140,9 @@ void double_blocked_potrf_impl(mifact_handle h, T* A, int lda_i, int n, int nb, 
 }
 
 template void double_blocked_potrf_impl<float>(mifact_handle, float*, int, int, int, int, int*,
-                                              bool, LowPrecision);
+                                              bool, LowPrecision, float*);
 template void double_blocked_potrf_impl<double>(mifact_handle, double*, int, int, int, int, int*,
-                                               bool, LowPrecision);
+                                               bool, LowPrecision, double*);
 
 namespace {
 
@@ -199,21 +206,109 @@ void getrf_panel_impl(mifact_handle h, int m, int n, T* A, int lda, int* ipiv, i
 static void potrf_f32(mifact_handle h, float* A, int lda, int n, int nb, int b, int* info,
                       bool dbtrsm)
 {
-    mifact::double_blocked_potrf_impl<float>(h, A, lda, n, nb, b, info, dbtrsm, mifact::kPrecFP32);
+    mifact::double_blocked_potrf_impl<float>(h, A, lda, n, nb, b, info, dbtrsm, mifact::kPrecFP32,
+                                             nullptr);
 }
 
 static void potrf_f64(mifact_handle h, double* A, int lda, int n, int nb, int b, int* info,
                       bool dbtrsm)
 {
     mifact::double_blocked_potrf_impl<double>(h, A, lda, n, nb, b, info, dbtrsm,
-                                              mifact::kPrecFP32);
+                                              mifact::kPrecFP32, nullptr);
 }
+
+// ------------------------------------------------------------ allocation-free POTRF
+//
+// Scratch of double_blocked_potrf_noalloc: the inverses of the 64 x 64
+// diagonal blocks of one inner panel, (b + 64) x 64 floats for inner blocks
+// b <= kNoAllocMaxB (1 MiB on every GPU the library runs on, held until the
+// process ends).  Every call on a GPU shares that GPU's copy: calls that
+// overlap on one GPU race.  Nothing in it needs to start at zero (each
+// block inverse is written by the diagonal factorization before the panel
+// solve reads it).
+namespace {
+constexpr int kNoAllocMaxB = 4096;
+constexpr size_t kScratchFloats = static_cast<size_t>(kNoAllocMaxB + 64) * 64;
+constexpr int kMaxDevices = 64;
+}  // namespace
+
+static __device__ __align__(256) float scratch[kScratchFloats];
+
+namespace {
+std::mutex g_scratch_mutex;
+float* g_scratch_ptr[kMaxDevices] = {};
+
+// Device address of `scratch` on the current device (resolved once per
+// device; the array's name is a host-side stand-in that must not be passed
+// to kernels).  Returns null on failure.
+float* scratch_address(mifact_handle h)
+{
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= kMaxDevices) {
+        mifact::report_status(h, mifact_status_hip_error, "noalloc: current device");
+        return nullptr;
+    }
+    std::lock_guard<std::mutex> lock(g_scratch_mutex);
+    if (g_scratch_ptr[dev] == nullptr) {
+        void* ptr = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || ptr == nullptr) {
+            mifact::report_status(h, mifact_status_hip_error,
+                                  "noalloc: hipGetSymbolAddress(scratch) failed");
+            return nullptr;
+        }
+        g_scratch_ptr[dev] = static_cast<float*>(ptr);
+    }
+    return g_scratch_ptr[dev];
+}
+
+__global__ void set_info_kernel(int* info, int v) { *info = v; }
+}  // namespace
+
+namespace mifact {
+// Resolves the allocation-free entry point's scratch for the current device
+// (called by mifact_create_handle, the library's set-up call, so the entry
+// point itself finds the address cached).
+bool noalloc_setup(mifact_handle h) { return scratch_address(h) != nullptr; }
+}  // namespace mifact
 
 extern "C" {
 
 void double_blocked_potrf(mifact_handle handle, float* A, int lda, int n, int nb, int b, int* info)
 {
     potrf_f32(handle, A, lda, n, nb, b, info, false);
+}
+
+void double_blocked_potrf_noalloc(mifact_handle handle, float* A, int lda, int n, int nb, int b,
+                                  int* info)
+{
+    if (handle == nullptr) {
+        mifact::report_status(nullptr, mifact_status_invalid_handle,
+                              "double_blocked_potrf_noalloc: null handle");
+        return;
+    }
+    if (n <= 0) return;
+    // the reference's block-size normalization, then the scratch bound
+    int nb_n = nb <= 0 ? n : nb;
+    int b_n = b <= 0 ? nb_n : b;
+    if (b_n > nb_n) b_n = nb_n;
+    if (b_n > n) b_n = n;       // no inner panel is wider than n
+    if (b_n > kNoAllocMaxB) {
+        std::fprintf(stderr,
+                     "double_blocked_potrf_noalloc: inner block %d exceeds the %d its "
+                     "fixed scratch holds\n", b_n, kNoAllocMaxB);
+        mifact::report_status(handle, mifact_status_invalid_value,
+                              "double_blocked_potrf_noalloc: inner block too large");
+        if (info != nullptr) {
+            hipLaunchKernelGGL(set_info_kernel, dim3(1), dim3(1), 0, handle->stream, info, -1);
+            (void)hipGetLastError();
+        }
+        return;
+    }
+    float* inv = scratch_address(handle);
+    if (inv == nullptr) return;
+    mifact::double_blocked_potrf_impl<float>(handle, A, lda, n, nb, b, info, false,
+                                             mifact::kPrecFP32, inv);
 }
 
 void double_blocked_potrf_dbtrsm(mifact_handle handle, float* A, int lda, int n, int nb, int b,

@@ -39,6 +39,21 @@
 extern "C" {
 #endif
 
+/* Allocation-free variant of double_blocked_potrf (no counterpart in the
+ * reference; for callers that cannot allocate device memory: code under HIP
+ * graph capture, tight serving loops, harnesses that forbid hipMalloc).
+ * Same arguments, schedule and results as double_blocked_potrf (FP32), but
+ * the diagonal-block inverses live in a fixed 1 MiB device array of the
+ * library instead of the handle's workspace, so the call performs no
+ * allocation of any kind.  Limit: inner block b <= 4096 after the usual
+ * normalization; a larger one prints an error, records
+ * mifact_status_invalid_value on the handle, sets *info = -1 and leaves A
+ * untouched.  All calls on one GPU share that array: calls that overlap on
+ * one GPU (other streams, other threads) race.  The handle's stage profiler
+ * must be off. */
+void double_blocked_potrf_noalloc(mifact_handle handle, float* A, int lda, int n, int nb, int b,
+                                  int* info);
+
 /* reference :9-15 */
 void double_blocked_potrf(mifact_handle handle, float* A, int lda, int n,
                           int nb, int b, int* info);
