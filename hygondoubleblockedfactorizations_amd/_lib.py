@@ -45,11 +45,21 @@ SIGNATURES = {
     "double_blocked_potrf_double_kblocked_syrk_adaptive_gemm": [P, P, I, I, I, I, I, P],
     "double_blocked_potrf_double_kblocked_syrk_adaptive_gemm_configured":
         [P, P, I, I, I, I, I, I, I, I, P],
+    # potrf/recursive_potrf.h
+    "recursive_potrf": [P, P, P, I, I, I, P],
+    "recursive_potrf_tiled_trsm": [P, P, P, I, I, I, P],
+    "recursive_potrf_double": [P, P, P, I, I, I, P],
+    "recursive_potrf_mixed_driver": [P, P, P, I, I, I, P],
     # getrf/double_blocked_getrf.h
     "double_blocked_getrf": [P, P, I, I, I, I, I, I, P, P],
     "double_blocked_getrf_double": [P, P, I, I, I, I, I, I, P, P],
     "double_blocked_getrf_batched_pivot": [P, P, I, I, I, I, I, I, I, P, P, P],
     "double_blocked_getrf_double_batched_pivot": [P, P, I, I, I, I, I, I, I, P, P, P],
+    # getrf/getrf_pivot_kernels.h
+    "launch_apply_getrf_panel_pivots": [P, I, I, I, I, P, P],
+    "launch_apply_getrf_panel_pivots_double": [P, I, I, I, I, P, P],
+    "launch_apply_getrf_permutation": [P, I, I, P, P, I, P, I, P],
+    "launch_apply_getrf_permutation_double": [P, I, I, P, P, I, P, I, P],
     # getrf/double_blocked_getrf_npvt.h
     "double_blocked_getrf_npvt": [P, P, I, I, I, I, P],
     "double_blocked_getrf_npvt_double": [P, P, I, I, I, I, P],

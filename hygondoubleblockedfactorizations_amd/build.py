@@ -33,10 +33,16 @@ SOURCES = [
     "panel.hip",
     "pivot.hip",
     "potrf.cpp",
+    "recursive.cpp",
     "getrf.cpp",
     "refine.hip",
     "batched.hip",
 ]
+
+# the rocblas_handle compatibility layer: its own library (the reference's
+# names, forwarding to libmifact.so), never linked into libmifact.so
+COMPAT_SOURCE = "compat_rocblas.cpp"
+COMPAT_LIB = os.path.join(LIBDIR, "libmifact_rocblas.so")
 
 COMMON_FLAGS = [
     "-O3", "-std=c++20", "-fPIC", f"--offload-arch={ARCH}",
@@ -129,7 +135,27 @@ def build(verbose: bool = True) -> str:
     os.replace(tmp, LIB)
     if verbose:
         print(f"built {LIB}")
+    build_compat(verbose)
     return LIB
+
+
+def build_compat(verbose: bool = True) -> str:
+    """libmifact_rocblas.so: host-only forwarding code, linked to rocBLAS
+    (rocblas_get_stream) and libdl."""
+    src = os.path.join(CSRC, COMPAT_SOURCE)
+    tmp = COMPAT_LIB + ".tmp"
+    rocm = os.environ.get("ROCM_PATH", "/opt/rocm")
+    cmd = [HIPCC, "-O2", "-std=c++20", "-fPIC", "-shared", f"-I{INCLUDE}",
+           f"-I{os.path.join(rocm, 'include')}", "-D__HIP_PLATFORM_AMD__", src, "-o", tmp,
+           f"-L{os.path.join(rocm, 'lib')}", "-lrocblas", "-ldl",
+           f"-Wl,-rpath,{os.path.join(rocm, 'lib')}"]
+    res = subprocess.run(cmd, capture_output=True, text=True)
+    if res.returncode != 0:
+        raise RuntimeError(f"compat build failed:\n{res.stdout}\n{res.stderr}")
+    os.replace(tmp, COMPAT_LIB)
+    if verbose:
+        print(f"built {COMPAT_LIB}")
+    return COMPAT_LIB
 
 
 if __name__ == "__main__":

@@ -727,7 +727,8 @@ void clear_int(mifact_handle h, int* p)
 // (info = -2).
 __device__ inline bool leaf_failed(const int* status)
 {
-    return __hip_atomic_load(status, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0;
+    return status != nullptr
+        && __hip_atomic_load(status, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0;
 }
 
 __global__ void build_moves_kernel(int p0, int K, const int* __restrict__ ipiv,
@@ -891,6 +892,132 @@ void apply_row_moves(mifact_handle h, T* A, long long lda, int c_begin, int c_en
     apply_moves<T>(h, A, lda, c_begin, c_end, nm, src, dst, tmp, tmp_elems);
 }
 
+// ------------------------------------------------------------------ pivot launchers
+//
+// The reference's stand-alone pivot launchers (include/getrf/
+// getrf_pivot_kernels.h:9-43, src/getrf/getrf_pivot_kernels.hip:14-206):
+// stream-ordered, no handle, no allocation.
+
+// Launch errors exit, as the reference's HIP_CHECK does (error_check.h:9-28).
+inline void report_launch(const char* what)
+{
+    const hipError_t e = hipGetLastError();
+    if (e != hipSuccess) report_hip(nullptr, e, what, __FILE__, __LINE__);
+}
+
+constexpr int kPanelChunk = 256;   // pivots composed per step
+template <typename T> constexpr int kPanelCols = 64 / sizeof(T);   // columns per workgroup
+
+// LAPACK laswp of one panel's local pivots over the columns outside the
+// panel: swaps rows panel + k <-> panel + pivots[k] - 1, k = 0 .. ps - 1 in
+// order.  A workgroup owns 16 / 8 columns (FP32 / FP64); pivots are taken kPanelChunk
+// at a time: the chunk's swaps are composed into a move list in LDS (each
+// candidate row's source found by undoing the chunk's swaps in reverse),
+// the moved entries of the workgroup's columns are gathered into LDS, then
+// scattered.  Chunks run in order, so the composition equals the sequence.
+template <typename T>
+__global__ __launch_bounds__(256) void panel_pivots_kernel(T* A, long long lda, int n, int panel,
+                                                           int ps, const int* __restrict__ piv)
+{
+    __shared__ int sp[kPanelChunk];
+    __shared__ int src[2 * kPanelChunk];
+    __shared__ int dst[2 * kPanelChunk];
+    __shared__ T vals[kPanelCols<T>][2 * kPanelChunk];   // 32 KiB
+    const int tid = threadIdx.x;
+    const int ext = n - ps;                       // columns outside the panel
+    const int e0 = blockIdx.x * kPanelCols<T>;
+    const int ncols = (ext - e0) < kPanelCols<T> ? (ext - e0) : kPanelCols<T>;
+    for (int k0 = 0; k0 < ps; k0 += kPanelChunk) {
+        const int kc = (ps - k0) < kPanelChunk ? (ps - k0) : kPanelChunk;
+        for (int t = tid; t < kc; t += 256) sp[t] = piv[k0 + t] - 1 - k0;  // chunk-local
+        __syncthreads();
+        for (int t = tid; t < 2 * kc; t += 256) {
+            const int d = t < kc ? t : sp[t - kc];
+            int pos = d;
+            for (int k = kc - 1; k >= 0; --k) {
+                const int pk = sp[k];
+                if (pos == k) pos = pk;
+                else if (pos == pk) pos = k;
+            }
+            src[t] = (pos != d) ? panel + k0 + pos : -1;
+            dst[t] = panel + k0 + d;
+        }
+        __syncthreads();
+        for (int e = tid; e < ncols * 2 * kc; e += 256) {
+            const int c = e / (2 * kc), t = e % (2 * kc);
+            const int ec = e0 + c;
+            const int col = ec < panel ? ec : ec + ps;
+            if (src[t] >= 0) vals[c][t] = A[off(src[t], col, lda)];
+        }
+        __syncthreads();
+        for (int e = tid; e < ncols * 2 * kc; e += 256) {
+            const int c = e / (2 * kc), t = e % (2 * kc);
+            const int ec = e0 + c;
+            const int col = ec < panel ? ec : ec + ps;
+            if (src[t] >= 0) A[off(dst[t], col, lda)] = vals[c][t];
+        }
+        __syncthreads();
+    }
+}
+
+__global__ void globalize_pivots_kernel(int* piv, int panel, int ps)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < ps) piv[i] += panel;
+}
+
+template <typename T>
+void launch_panel_pivots(T* A, int lda, int n, int panel, int ps, int* pivots, hipStream_t s)
+{
+    if (ps <= 0) return;
+    const int ext = n - ps;
+    if (ext > 0) {
+        hipLaunchKernelGGL(panel_pivots_kernel<T>, dim3(ceil_div(ext, kPanelCols<T>)), dim3(256), 0,
+                           s, A, static_cast<long long>(lda), n, panel, ps, pivots);
+        report_launch("launch_apply_getrf_panel_pivots");
+    }
+    hipLaunchKernelGGL(globalize_pivots_kernel, dim3(ceil_div(ps, 256)), dim3(256), 0, s, pivots,
+                       panel, ps);
+    report_launch("launch_apply_getrf_panel_pivots");
+}
+
+// Row moves over columns [0, column_count): row targets[m] receives the
+// pre-call entry of row sources[m].  The LDS kernel when a column of the
+// move list fits in LDS, else the caller's temporary in column tiles (the
+// reference's arrangement).
+template <typename T>
+void launch_permutation(T* A, int lda, int column_count, const int* sources, const int* targets,
+                        int move_count, T* temporary, int column_tile, hipStream_t s)
+{
+    if (column_count <= 0 || move_count <= 0) return;
+    constexpr size_t kLdsBudget = 64 * 1024;
+    const size_t per_col = static_cast<size_t>(move_count) * sizeof(T);
+    if (per_col <= kLdsBudget) {
+        int CT = static_cast<int>(kLdsBudget / per_col);
+        if (CT > 16) CT = 16;
+        hipLaunchKernelGGL(apply_moves_lds_kernel<T>, dim3(ceil_div(column_count, CT)), dim3(256),
+                           static_cast<size_t>(CT) * per_col, s, A, static_cast<long long>(lda), 0,
+                           column_count, CT, move_count, sources, targets,
+                           static_cast<const int*>(nullptr));
+        report_launch("launch_apply_getrf_permutation");
+        return;
+    }
+    if (temporary == nullptr || column_tile <= 0) return;
+    for (int c0 = 0; c0 < column_count; c0 += column_tile) {
+        const int nc = (column_count - c0) < column_tile ? (column_count - c0) : column_tile;
+        const size_t total = static_cast<size_t>(nc) * move_count;
+        const int blocks = static_cast<int>(total / 256 + 1 < 65535 ? total / 256 + 1 : 65535);
+        hipLaunchKernelGGL(gather_moves_kernel<T>, dim3(blocks), dim3(256), 0, s, A,
+                           static_cast<long long>(lda), c0, nc, move_count, sources, temporary,
+                           static_cast<const int*>(nullptr));
+        report_launch("launch_apply_getrf_permutation");
+        hipLaunchKernelGGL(scatter_moves_kernel<T>, dim3(blocks), dim3(256), 0, s, A,
+                           static_cast<long long>(lda), c0, nc, move_count, sources, targets,
+                           temporary, static_cast<const int*>(nullptr));
+        report_launch("launch_apply_getrf_permutation");
+    }
+}
+
 template void getf2_leaf<float>(mifact_handle, float*, long long, int, int, int, int*, int*, int*,
                                 float*, float*, int*, int*);
 template void getf2_leaf<double>(mifact_handle, double*, long long, int, int, int, int*, int*,
@@ -907,3 +1034,36 @@ template void apply_row_moves<double>(mifact_handle, double*, long long, int, in
                                       const int*, const int*, const int*, double*, long long);
 
 }  // namespace mifact
+
+extern "C" {
+
+void launch_apply_getrf_panel_pivots(float* A, int lda, int n, int panel, int panel_size,
+                                     int* pivots, hipStream_t stream)
+{
+    mifact::launch_panel_pivots<float>(A, lda, n, panel, panel_size, pivots, stream);
+}
+
+void launch_apply_getrf_panel_pivots_double(double* A, int lda, int n, int panel, int panel_size,
+                                            int* pivots, hipStream_t stream)
+{
+    mifact::launch_panel_pivots<double>(A, lda, n, panel, panel_size, pivots, stream);
+}
+
+void launch_apply_getrf_permutation(float* A, int lda, int column_count, const int* sources,
+                                    const int* targets, int move_count, float* temporary,
+                                    int column_tile, hipStream_t stream)
+{
+    mifact::launch_permutation<float>(A, lda, column_count, sources, targets, move_count,
+                                      temporary, column_tile, stream);
+}
+
+void launch_apply_getrf_permutation_double(double* A, int lda, int column_count,
+                                           const int* sources, const int* targets,
+                                           int move_count, double* temporary, int column_tile,
+                                           hipStream_t stream)
+{
+    mifact::launch_permutation<double>(A, lda, column_count, sources, targets, move_count,
+                                       temporary, column_tile, stream);
+}
+
+}  // extern "C"

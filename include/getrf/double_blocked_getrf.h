@@ -33,29 +33,11 @@
 #pragma once
 
 #include "mifact/handle.h"
+#include "getrf/getrf_types.h"
 
 #ifdef __cplusplus
 extern "C" {
 #endif
-
-/* reference :9-19 (layout unchanged) */
-typedef struct getrf_batched_pivot_workspace {
-    int* device_sources;
-    int* device_targets;
-    void* device_swap_buffer;
-    int* host_pivots;
-    int* host_permutation;
-    int* host_sources;
-    int* host_targets;
-    int capacity;
-    int column_tile;
-} getrf_batched_pivot_workspace;
-
-/* reference :21-24 */
-typedef enum getrf_outer_trsm_mode {
-    getrf_outer_trsm_direct = 0,
-    getrf_outer_trsm_iterative = 1
-} getrf_outer_trsm_mode;
 
 /* reference :26-35 */
 void double_blocked_getrf(mifact_handle handle, float* A, int lda, int n,

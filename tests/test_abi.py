@@ -14,9 +14,13 @@ from hygondoubleblockedfactorizations_amd import _lib
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 
 
-def declared_functions():
+def declared_functions(compat=False):
+    """Functions the native headers declare (compat=True: the rocblas_handle
+    compatibility header instead)."""
     names = set()
     for path in glob.glob(os.path.join(ROOT, "include", "**", "*.h"), recursive=True):
+        if (os.sep + "compat" + os.sep in path) != compat:
+            continue
         text = open(path).read()
         text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
         text = re.sub(r"//[^\n]*", "", text)
@@ -30,8 +34,11 @@ def declared_functions():
     return names
 
 
-def exported_symbols():
-    out = subprocess.run(["nm", "-D", "--defined-only", _lib.LIB_PATH],
+COMPAT_LIB = os.path.join(os.path.dirname(_lib.LIB_PATH), "libmifact_rocblas.so")
+
+
+def exported_symbols(path=None):
+    out = subprocess.run(["nm", "-D", "--defined-only", path or _lib.LIB_PATH],
                          capture_output=True, text=True, check=True).stdout
     return {line.split()[-1] for line in out.splitlines() if line.strip()}
 
@@ -91,3 +98,17 @@ def test_missing_library_fails_loudly(tmp_path):
             _lib.load(str(tmp_path / "absent.so"))
     finally:
         _lib._lib = saved
+
+
+def test_compat_library_exports_reference_prototypes():
+    """libmifact_rocblas.so exports every prototype of the compatibility
+    header (the reference's names, taking rocblas_handle); it links rocBLAS
+    (for rocblas_get_stream only) while libmifact.so does not, and every
+    name it forwards exists in libmifact.so."""
+    declared = declared_functions(compat=True)
+    assert len(declared) >= 30
+    assert not (declared - exported_symbols(COMPAT_LIB))
+    assert not (declared - exported_symbols())
+    out = subprocess.run(["ldd", COMPAT_LIB], capture_output=True, text=True).stdout
+    assert "rocblas" in out
+    assert "libmifact.so" not in out           # loaded privately at run time
