@@ -1,0 +1,283 @@
+// Host replay of the batched LU kernel's address arithmetic
+// (hygondoubleblockedfactorizations_amd/csrc/batched.hip: order_kernel,
+// batched_kernel<kPrecFP32, true>, lu_matrix, tile_update<kPrecFP32, false>).
+//
+// Every phase between two workgroup barriers runs for tid = 0..255 in turn;
+// the wave butterflies run lane by lane.  Every access to the packed matrix
+// buffer, to ipiv and to the workgroup arrays goes through a checked
+// accessor that counts accesses outside (a) the whole buffer, (b) the
+// current matrix's lda x n block, (c) its n x n part.  Values are computed
+// exactly as the kernel computes them (FP32), so the pivot sequence (which
+// decides the addresses of the row exchanges) is the kernel's.
+//
+// Input (little endian): int32 batch, int64 total, int32 n[batch],
+// int32 lda[batch], int64 offsets[batch], int64 ipiv_offsets[batch],
+// float flat[total].  Output: float flat[total], int32 ipiv[sum max(n,1)],
+// int32 info[batch].  Prints the count of out-of-range accesses; exit 1 if
+// any.
+#include <algorithm>
+#include <cfloat>
+#include <cmath>
+#include <cstdint>
+#include <cstdio>
+#include <vector>
+
+namespace {
+
+constexpr int kNB = 32, kT = 64, kMaxOrder = 4096, kBuckets = 256, kThreads = 256;
+
+struct Checked {
+    std::vector<float> flat;
+    std::vector<int> ipiv;
+    long long lo = 0, hi = 0, plo = 0, phi = 0;  // current matrix / ipiv range
+    long long base = 0, lda = 0;
+    int n = 0;
+    long long violations = 0;
+    float sink = 0.0f;
+    int isink = 0;
+
+    void report(const char* what, long long a, long long b)
+    {
+        if (violations++ < 20) std::printf("out of range: %s (%lld, %lld)\n", what, a, b);
+    }
+    float& A(long long r, long long c)
+    {
+        const long long i = base + r + c * lda;
+        if (r < 0 || r >= n || c < 0 || c >= n) report("outside n x n", r, c);
+        if (i < lo || i >= hi) report("outside lda x n block", r, c);
+        if (i < 0 || i >= static_cast<long long>(flat.size())) {
+            report("outside buffer", r, c);
+            return sink;
+        }
+        return flat[i];
+    }
+    int& piv(long long j)
+    {
+        const long long i = plo + j;
+        if (i < plo || i >= phi || i >= static_cast<long long>(ipiv.size())) {
+            report("ipiv", j, 0);
+            return isink;
+        }
+        return ipiv[i];
+    }
+};
+
+template <typename T, int R, int C>
+struct Lds {
+    T v[R][C];
+    long long* violations;
+    T& at(int r, int c)
+    {
+        static T s;
+        if (r < 0 || r >= R || c < 0 || c >= C) {
+            ++*violations;
+            return s;
+        }
+        return v[r][c];
+    }
+};
+
+// C(c1 + r0 .., c1 + c0 ..) -= L21 U12 on one 64 x 64 tile (the MFMA
+// accumulator layout of v_mfma_f32_32x32x2_f32: lane l holds column l % 32,
+// rows 8 (r / 4) + 4 (l / 32) + r % 4).
+void tile_update(Checked& M, int c1, int k, int m, int r0, int c0, int kb)
+{
+    static Lds<float, kNB, kT + 1> sa, sb;
+    sa.violations = sb.violations = &M.violations;
+    for (int tid = 0; tid < kThreads; ++tid)
+        for (int e = tid; e < kNB * kT; e += kThreads) {
+            const int kk = e / kT, i = e % kT;
+            float va = 0.0f, vb = 0.0f;
+            if (kk < kb) {
+                if (r0 + i < m) va = M.A(c1 + r0 + i, k + kk);
+                if (c0 + i < m) vb = M.A(k + kk, c1 + c0 + i);
+            }
+            sa.at(kk, i) = va;
+            sb.at(kk, i) = vb;
+        }
+    for (int tid = 0; tid < kThreads; ++tid) {
+        const int lane = tid & 63, wave = tid >> 6, wm = wave & 1, wn = wave >> 1;
+        const int col = wn * 32 + (lane & 31);
+        const int gc = c0 + col;
+        if (gc >= m) continue;
+        for (int r = 0; r < 16; ++r) {
+            const int lr = wm * 32 + 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3);
+            float acc = 0.0f;
+            for (int kk = 0; kk < kNB; ++kk) acc = std::fma(sa.at(kk, lr), sb.at(kk, col), acc);
+            if (r0 + lr < m) M.A(c1 + r0 + lr, c1 + gc) -= acc;
+        }
+    }
+}
+
+void lu_matrix(Checked& M, int n, int* info)
+{
+    float wv[4];
+    int wi[4];
+    Lds<float, 1, kNB> prow;
+    Lds<float, kNB, kNB + 1> l11;
+    prow.violations = l11.violations = &M.violations;
+    for (int k = 0; k < n; k += kNB) {
+        const int kb = std::min(kNB, n - k);
+        for (int j = k; j < k + kb; ++j) {
+            float bv[kThreads];
+            int bi[kThreads];
+            for (int tid = 0; tid < kThreads; ++tid) {
+                bv[tid] = -1.0f;
+                bi[tid] = n;
+                for (int i = j + tid; i < n; i += kThreads) {
+                    const float v = std::fabs(M.A(i, j));
+                    if (v > bv[tid]) {
+                        bv[tid] = v;
+                        bi[tid] = i;
+                    }
+                }
+            }
+            for (int w = 0; w < 4; ++w) {
+                for (int o = 32; o > 0; o >>= 1) {
+                    float nv[64];
+                    int ni[64];
+                    for (int l = 0; l < 64; ++l) {
+                        float v = bv[w * 64 + l];
+                        int i = bi[w * 64 + l];
+                        const float ov = bv[w * 64 + (l ^ o)];
+                        const int oi = bi[w * 64 + (l ^ o)];
+                        if (ov > v || (ov == v && oi < i)) {
+                            v = ov;
+                            i = oi;
+                        }
+                        nv[l] = v;
+                        ni[l] = i;
+                    }
+                    for (int l = 0; l < 64; ++l) {
+                        bv[w * 64 + l] = nv[l];
+                        bi[w * 64 + l] = ni[l];
+                    }
+                }
+                wv[w] = bv[w * 64];
+                wi[w] = bi[w * 64];
+            }
+            float pv = wv[0];
+            int p = wi[0];
+            for (int w = 1; w < 4; ++w)
+                if (wv[w] > pv || (wv[w] == pv && wi[w] < p)) {
+                    pv = wv[w];
+                    p = wi[w];
+                }
+            if (p >= n) p = j;
+            M.piv(j) = p + 1;
+            const float piv = M.A(p, j);
+            const bool zero = piv == 0.0f;
+            if (zero && *info == 0) *info = j + 1;
+            if (p != j)
+                for (int c = 0; c < n; ++c) std::swap(M.A(j, c), M.A(p, c));
+            for (int t = 0; t < kNB; ++t)
+                prow.at(0, t) = (j + 1 + t < k + kb) ? M.A(j, j + 1 + t) : 0.0f;
+            if (!zero) {
+                const bool recip = std::fabs(piv) >= FLT_MIN;
+                const float rinv = 1.0f / piv;
+                const int ncol = k + kb - j - 1;
+                for (int i = j + 1; i < n; ++i) {
+                    float l = M.A(i, j);
+                    l = recip ? l * rinv : l / piv;
+                    M.A(i, j) = l;
+                    for (int c = 0; c < ncol; ++c) M.A(i, j + 1 + c) -= l * prow.at(0, c);
+                }
+            }
+        }
+        const int c1 = k + kb, m = n - c1;
+        if (m <= 0) continue;
+        for (int e = 0; e < kNB * kNB; ++e) {
+            const int i = e % kNB, jj = e / kNB;
+            l11.at(i, jj) = (i < kb && jj < kb) ? M.A(k + i, k + jj) : 0.0f;
+        }
+        for (int c = c1; c < n; ++c) {
+            float x[kNB];
+            for (int i = 0; i < kNB; ++i) x[i] = i < kb ? M.A(k + i, c) : 0.0f;
+            for (int i = 1; i < kNB; ++i) {
+                float v = x[i];
+                for (int q = 0; q < i; ++q) v -= l11.at(i, q) * x[q];
+                x[i] = v;
+            }
+            for (int i = 0; i < kb; ++i) M.A(k + i, c) = x[i];
+        }
+        const int tiles = (m + kT - 1) / kT;
+        for (int tj = 0; tj < tiles; ++tj)
+            for (int ti = 0; ti < tiles; ++ti) tile_update(M, c1, k, m, ti * kT, tj * kT, kb);
+    }
+}
+
+template <typename T>
+bool get(FILE* f, T* p, size_t count)
+{
+    return std::fread(p, sizeof(T), count, f) == count;
+}
+
+}  // namespace
+
+int main(int argc, char** argv)
+{
+    if (argc != 3) {
+        std::fprintf(stderr, "usage: %s input output\n", argv[0]);
+        return 2;
+    }
+    FILE* f = std::fopen(argv[1], "rb");
+    if (f == nullptr) return 2;
+    int batch = 0;
+    long long total = 0;
+    if (!get(f, &batch, 1) || !get(f, &total, 1) || batch <= 0 || total <= 0) return 2;
+    std::vector<int> n(batch), lda(batch);
+    std::vector<long long> off(batch), poff(batch);
+    Checked M;
+    M.flat.resize(total);
+    if (!get(f, n.data(), batch) || !get(f, lda.data(), batch) || !get(f, off.data(), batch)
+        || !get(f, poff.data(), batch) || !get(f, M.flat.data(), total))
+        return 2;
+    std::fclose(f);
+    long long ptotal = 0;
+    for (int v : n) ptotal += std::max(v, 1);
+    M.ipiv.assign(ptotal, 0);
+
+    // order_kernel: counting sort on n / 16, largest first
+    std::vector<int> hist(kBuckets, 0), start(kBuckets, 0), order(batch, -1);
+    auto bucket = [&](int i) {
+        const int v = n[i] / 16;
+        return kBuckets - 1 - (v < 0 ? 0 : (v >= kBuckets ? kBuckets - 1 : v));
+    };
+    for (int i = 0; i < batch; ++i) ++hist[bucket(i)];
+    for (int b = 0, s = 0; b < kBuckets; ++b) {
+        start[b] = s;
+        s += hist[b];
+    }
+    for (int i = 0; i < batch; ++i) {
+        const int pos = start[bucket(i)]++;
+        if (pos < 0 || pos >= batch) M.report("order", pos, i);
+        else order[pos] = i;
+    }
+    std::vector<int> info(batch, 99);
+    for (int q = 0; q < batch; ++q) {
+        const int id = order[q];
+        if (id < 0 || id >= batch) {
+            M.report("queue id", q, id);
+            continue;
+        }
+        const int nn = n[id];
+        if (nn <= 0 || nn > kMaxOrder) continue;
+        M.base = off[id];
+        M.lda = lda[id];
+        M.n = nn;
+        M.lo = off[id];
+        M.hi = off[id] + static_cast<long long>(lda[id]) * nn;
+        M.plo = poff[id];
+        M.phi = poff[id] + nn;
+        info[id] = 0;
+        lu_matrix(M, nn, &info[id]);
+    }
+    std::printf("batch %d: out-of-range accesses %lld\n", batch, M.violations);
+    FILE* o = std::fopen(argv[2], "wb");
+    if (o == nullptr) return 2;
+    std::fwrite(M.flat.data(), sizeof(float), M.flat.size(), o);
+    std::fwrite(M.ipiv.data(), sizeof(int), M.ipiv.size(), o);
+    std::fwrite(info.data(), sizeof(int), info.size(), o);
+    std::fclose(o);
+    return M.violations != 0;
+}
