@@ -15,85 +15,219 @@ namespace mifact {
 
 constexpr int kLeaf = 64;  // leaf order of POTF2 / triangular inverse kernels
 
-// Row i of M = inv(L) for a lower-triangular s x s block held in LDS
-// (sL[k][j] = L(k, j)), computed by thread i from the last column to the
-// first: M(i,i) = 1/L(i,i) (1 when unit), M(i,j) = -sum_{k>j} M(i,k) L(k,j)
-// / L(j,j).  Every LDS read is a broadcast (same address on all lanes).
-template <typename T, bool UNIT>
-__device__ __attribute__((always_inline)) inline void inverse_row(int s, const T (*sL)[kLeaf + 1],
-                                                                 T (&m)[kLeaf])
+// One-wave leaf kernels keep their code small: a wave that runs through
+// straight-line code once pays an instruction-cache fill for every line of
+// it, so the column loops are runtime loops over groups of kGroup columns
+// with fixed register indices, and the registers rotate by kGroup between
+// groups (a "window" of W registers holds the live columns).
+constexpr int kGroup = 4;
+constexpr int kPadL = 64;                    // zero columns in front of each LDS row
+constexpr int kLdsLd = kPadL + 2 * kLeaf + 4;  // LDS row stride: padding, 64 columns, overhang
+
+template <typename T>
+using Vec4 = T __attribute__((ext_vector_type(4)));
+
+// value of v in lane `lane` (uniform)
+template <typename T>
+__device__ inline T lane_value(T v, int lane)
+{
+    if constexpr (sizeof(T) == 4) {
+        return __builtin_bit_cast(T, __builtin_amdgcn_readlane(__builtin_bit_cast(int, v), lane));
+    } else {
+        const long long b = __builtin_bit_cast(long long, v);
+        const int lo = __builtin_amdgcn_readlane(static_cast<int>(b & 0xffffffffLL), lane);
+        const int hi = __builtin_amdgcn_readlane(static_cast<int>(b >> 32), lane);
+        return __builtin_bit_cast(
+            T, (static_cast<long long>(hi) << 32) | static_cast<unsigned int>(lo));
+    }
+}
+
+// Row `ri` of an s x s block, columns 0..63, as 64 unconditional loads
+// (column index clamped into the block).  The loads are pinned to registers
+// before any select so that none of them becomes a branch around a load
+// (which would make them complete one at a time).
+template <typename T>
+__device__ __attribute__((always_inline)) inline void load_row(const T* __restrict__ A,
+                                                               long long lda, int s, int ri,
+                                                               T (&x)[kLeaf])
+{
+    const T* row = A + ri;
+#pragma unroll
+    for (int t = 0; t < kLeaf; ++t) x[t] = row[static_cast<size_t>(t < s ? t : s - 1) * lda];
+#pragma unroll
+    for (int t = 0; t < kLeaf; ++t) asm volatile("" : "+v"(x[t]));
+}
+
+// One phase of the right-looking Cholesky of a block padded with the
+// identity beyond s (so no column needs a size test): columns j0 + c of the
+// groups j0 = jbeg, jbeg + 4, ... < jend.  x[t] holds column j0 + t of this
+// lane's row.  The pivot of the next column is formed by its own lane
+// (x[c + 1] - l^2) while the other lanes' updates run, and one wave's LDS
+// operations complete in order, so no barrier is needed.  Finished columns
+// go to sL (row i at kPadL + column); r/ljj carry the current pivot; jf
+// becomes the first column whose pivot is not positive (ends the phase).
+template <typename T, int W>
+__device__ __attribute__((always_inline)) inline void chol_phase(int jbeg, int jend,
+                                                                 T (&x)[kLeaf], T* col,
+                                                                 T (*sL)[kLdsLd], T* rd, int& jf,
+                                                                 T& ljj, T& r)
 {
     const int i = threadIdx.x;
+    for (int j0 = jbeg; j0 < jend && jf == kLeaf; j0 += kGroup) {
 #pragma unroll
-    for (int j = kLeaf - 1; j >= 0; --j) {
-        if (j < s) {
-            // four partial sums break the dependent FMA chain
-            T acc[4] = {T(0), T(0), T(0), T(0)};
+        for (int c = 0; c < kGroup; ++c) {
+            const int j = j0 + c;
+            const T lij = (i > j) ? x[c] * r : (i == j ? ljj : T(0));
+            x[c] = lij;
+            rd[j] = r;
+            col[i] = lij;
+            const T dn = lane_value(x[c + 1] - lij * lij, (j + 1) & (kLeaf - 1));
+            jf = (!(dn > T(0)) && jf == kLeaf && j + 1 < kLeaf) ? j + 1 : jf;
+            const T ljj_n = sqrt(dn);
+            const T r_n = T(1) / ljj_n;
+            T cv[W];
 #pragma unroll
-            for (int k = j + 1; k < kLeaf; ++k) acc[(k - j - 1) & 3] += m[k] * sL[k][j];
-            const T sum = (acc[0] + acc[1]) + (acc[2] + acc[3]);
-            const T dj = UNIT ? T(1) : T(1) / sL[j][j];
-            m[j] = (i == j) ? dj : (i > j ? -sum * dj : T(0));
-        } else {
-            m[j] = T(0);
+            for (int q = 0; q < W; q += 4) {
+                const Vec4<T> v = *reinterpret_cast<const Vec4<T>*>(&col[j0 + q]);
+                cv[q] = v[0];
+                cv[q + 1] = v[1];
+                cv[q + 2] = v[2];
+                cv[q + 3] = v[3];
+            }
+#pragma unroll
+            for (int t = c + 1; t < W; ++t) x[t] -= lij * cv[t];
+            ljj = ljj_n;
+            r = r_n;
         }
+        *reinterpret_cast<Vec4<T>*>(&sL[i][kPadL + j0]) = Vec4<T>{x[0], x[1], x[2], x[3]};
+#pragma unroll
+        for (int t = 0; t < W - kGroup; ++t) x[t] = x[t + kGroup];
+#pragma unroll
+        for (int t = W - kGroup; t < W; ++t) x[t] = T(0);
+    }
+}
+
+// Inverse M = inv(L) of the 64 x 64 lower-triangular block in sL (identity
+// beyond s), row i per lane, from the last column to the first:
+// M(i, k) = (delta_ik - sum_{k' > k} M(i, k') L(k', k)) * rd[k]; after step k
+// the accumulators of every column j < k gain M(i, k) L(k, j) (row k of L,
+// read as a broadcast; the next row is fetched during the current step).
+// a[t] <-> column kt - t within a group; the front padding of sL covers the
+// window's columns below 0 (FP32 only: FP64 has no registers for the
+// prefetch).  Writes M(i, k) for i, k < s at inv + 64 k + i.  A unit
+// diagonal is given as rd = 1.
+template <typename T>
+__device__ __attribute__((always_inline)) inline void inverse_from_lds(int s,
+                                                                       const T (*sL)[kLdsLd],
+                                                                       const T* rd,
+                                                                       T* __restrict__ inv)
+{
+    constexpr int W = kLeaf;
+    constexpr bool kPrefetch = sizeof(T) == 4;
+    const int i = threadIdx.x;
+    T a[kLeaf];
+#pragma unroll
+    for (int t = 0; t < kLeaf; ++t) a[t] = T(0);
+    for (int kt = kLeaf - 1; kt >= 0; kt -= kGroup) {
+        T lv[W];
+#pragma unroll
+        for (int q = 0; q < W; q += 4) {
+            const Vec4<T> v = *reinterpret_cast<const Vec4<T>*>(&sL[kt][kPadL + kt - W + 1 + q]);
+            lv[q] = v[0];
+            lv[q + 1] = v[1];
+            lv[q + 2] = v[2];
+            lv[q + 3] = v[3];
+        }
+#pragma unroll
+        for (int c = 0; c < kGroup; ++c) {
+            const int k = kt - c;
+            T ln[W];
+            if (kPrefetch && c + 1 < kGroup) {
+#pragma unroll
+                for (int q = 0; q < W; q += 4) {
+                    const Vec4<T> v =
+                        *reinterpret_cast<const Vec4<T>*>(&sL[k - 1][kPadL + kt - W + 1 + q]);
+                    ln[q] = v[0];
+                    ln[q + 1] = v[1];
+                    ln[q + 2] = v[2];
+                    ln[q + 3] = v[3];
+                }
+            }
+            const T mk = (((i == k) ? T(1) : T(0)) - a[c]) * rd[k];
+            if (i < s && k < s) inv[k * kLeaf + i] = mk;
+#pragma unroll
+            for (int t = c + 1; t < W; ++t) a[t] += mk * lv[W - 1 - t];
+            if (c + 1 < kGroup) {
+                if (kPrefetch) {
+#pragma unroll
+                    for (int q = 0; q < W; ++q) lv[q] = ln[q];
+                } else {
+#pragma unroll
+                    for (int q = 0; q < W; q += 4) {
+                        const Vec4<T> v = *reinterpret_cast<const Vec4<T>*>(
+                            &sL[k - 1][kPadL + kt - W + 1 + q]);
+                        lv[q] = v[0];
+                        lv[q + 1] = v[1];
+                        lv[q + 2] = v[2];
+                        lv[q + 3] = v[3];
+                    }
+                }
+            }
+        }
+#pragma unroll
+        for (int t = 0; t < W - kGroup; ++t) a[t] = a[t + kGroup];
+#pragma unroll
+        for (int t = W - kGroup; t < W; ++t) a[t] = T(0);
     }
 }
 
 // ------------------------------------------------------------------ POTF2
-// Unblocked lower Cholesky of an s x s block (s <= 64) by one wave64: lane i
-// holds row i in registers; per column j the pivot is broadcast with a lane
-// shuffle, the scaled column goes through LDS and every lane updates its row
-// (right-looking; reciprocal scaling as LAPACK xPOTF2).  When inv is not
-// null the wave then writes inv(L) (column-major, ld 64): the panel TRSM
-// applies it with an MFMA multiply instead of a substitution.
+// Unblocked lower Cholesky of an s x s block (s <= 64) by one wave64: lane
+// i holds row i in registers (rows beyond s padded with the identity);
+// per column the pivot is a square root and the column is scaled by its
+// reciprocal (LAPACK xPOTF2 arithmetic), then every lane updates its row
+// (right-looking).  A non-positive or NaN pivot records col_offset + j + 1
+// in *info (first failure in launch order), leaves the columns before it
+// factored and everything from it on unchanged.  When inv is not null the
+// wave then writes inv(L) (column-major, ld 64): the panel TRSM applies it
+// with an MFMA multiply instead of a substitution.
 template <typename T>
 __global__ __launch_bounds__(64) void potf2_kernel(int s, T* __restrict__ A, long long lda,
                                                    T* __restrict__ inv, int* __restrict__ info,
                                                    int col_offset)
 {
-    __shared__ T col[kLeaf];
-    __shared__ T sL[kLeaf][kLeaf + 1];
+    __shared__ __attribute__((aligned(16))) T col[2 * kLeaf];
+    __shared__ __attribute__((aligned(16))) T sL[kLeaf][kLdsLd];
+    __shared__ T rd[kLeaf];
     const int i = threadIdx.x;
     T x[kLeaf];
+    load_row(A, lda, s, i < s ? i : s - 1, x);
 #pragma unroll
-    for (int k = 0; k < kLeaf; ++k) x[k] = (i < s && k <= i) ? A[off(i, k, lda)] : T(0);
-    bool failed = false;
+    for (int t = 0; t < kLeaf; ++t)
+        x[t] = (i < s) ? (t <= i ? x[t] : T(0)) : (t == i ? T(1) : T(0));
+    col[kLeaf + i] = T(0);
 #pragma unroll
-    for (int j = 0; j < kLeaf; ++j) {
-        if (!failed && j < s) {
-            const T d = __shfl(x[j], j);
-            if (!(d > T(0))) {
-                failed = true;
-                if (i == 0 && info != nullptr) atomicCAS(info, 0, col_offset + j + 1);
-            } else {
-                const T ljj = sqrt(d);
-                const T r = T(1) / ljj;
-                if (i == j) {
-                    x[j] = ljj;
-                } else if (i > j) {
-                    x[j] *= r;
-                }
-                col[i] = x[j];
-                __syncthreads();
-#pragma unroll
-                for (int k = j + 1; k < kLeaf; ++k) x[k] -= x[j] * col[k];
-                __syncthreads();
-            }
-        }
+    for (int t = 0; t < kPadL; t += 4)
+        *reinterpret_cast<Vec4<T>*>(&sL[i][t]) = Vec4<T>{T(0), T(0), T(0), T(0)};
+    int jf = kLeaf;
+    const T d0 = lane_value(x[0], 0);
+    if (!(d0 > T(0))) jf = 0;
+    T ljj = sqrt(d0);
+    T r = T(1) / ljj;
+    chol_phase<T, 64>(0, 16, x, col, sL, rd, jf, ljj, r);
+    chol_phase<T, 48>(16, 32, x, col, sL, rd, jf, ljj, r);
+    chol_phase<T, 32>(32, 48, x, col, sL, rd, jf, ljj, r);
+    chol_phase<T, 16>(48, kLeaf, x, col, sL, rd, jf, ljj, r);
+    // row i's finished columns (all, or those before a failing pivot)
+    const int done = jf < s ? jf : s;
+    if (i < s)
+        for (int t = 0; t <= i && t < done; ++t) A[off(i, t, lda)] = sL[i][kPadL + t];
+    if (jf < s) {
+        if (i == 0 && info != nullptr) atomicCAS(info, 0, col_offset + jf + 1);
+        return;
     }
-#pragma unroll
-    for (int k = 0; k < kLeaf; ++k)
-        if (i < s && k <= i) A[off(i, k, lda)] = x[k];
-    if (inv == nullptr || failed) return;
-#pragma unroll
-    for (int k = 0; k < kLeaf; ++k) sL[i][k] = x[k];
-    __syncthreads();
-    T m[kLeaf];
-    inverse_row<T, false>(s, sL, m);
-#pragma unroll
-    for (int j = 0; j < kLeaf; ++j)
-        if (i < s && j < s) inv[j * kLeaf + i] = m[j];
+    if (inv == nullptr) return;
+    inverse_from_lds<T>(s, sL, rd, inv);
 }
 
 // Unpivoted LU of an s x s block (s <= 64) in one wave: lane i owns row i
@@ -159,32 +293,44 @@ void potf2_base(mifact_handle h, int s, T* A, long long lda, T* inv, int* info, 
 // inv + 64*64*b (column-major, ld 64).
 // UPPER: the block is read transposed (an upper-triangular U as the lower
 // U^T), so the result is inv(U^T) = inv(U)^T.
+// Lane k loads row k of the block (coalesced across lanes), the lower
+// triangle of L goes to LDS as rows (identity beyond s), then the inverse
+// is formed as in the POTF2 kernel.
 template <typename T, bool UNIT, bool UPPER>
 __global__ __launch_bounds__(64) void trtri_kernel(int n, const T* __restrict__ L, long long ldl,
                                                    T* __restrict__ inv)
 {
-    __shared__ T sL[kLeaf][kLeaf + 1];
+    __shared__ __attribute__((aligned(16))) T sL[kLeaf][kLdsLd];
+    __shared__ T rd[kLeaf];
     const int b = blockIdx.x;
     const int r0 = b * kLeaf;
     const int s = (n - r0) < kLeaf ? (n - r0) : kLeaf;
-    const int i = threadIdx.x;
+    const int k = threadIdx.x;
     const T* Lb = L + off(r0, r0, ldl);
-#pragma unroll 8
-    for (int k = 0; k < kLeaf; ++k) {
-        const bool in = i < s && k < s && k <= i;
-        if (UPPER) {
-            sL[i][k] = in ? Lb[off(k, i, ldl)] : T(0);  // U^T(i, k) = U(k, i)
-        } else {
-            sL[i][k] = in ? Lb[off(i, k, ldl)] : T(0);
+    T x[kLeaf];
+    load_row(Lb, ldl, s, k < s ? k : s - 1, x);   // x[t] = block(k, t)
+#pragma unroll
+    for (int t = 0; t < kPadL; t += 4)
+        *reinterpret_cast<Vec4<T>*>(&sL[k][t]) = Vec4<T>{T(0), T(0), T(0), T(0)};
+    if (UPPER) {
+        // U^T(t, k) = U(k, t): lane k's row of U is column k of U^T
+#pragma unroll
+        for (int t = 0; t < kLeaf; ++t)
+            sL[t][kPadL + k] = (k < s && t < s) ? (k <= t ? x[t] : T(0)) : (t == k ? T(1) : T(0));
+    } else {
+#pragma unroll
+        for (int t = 0; t < kLeaf; t += 4) {
+            Vec4<T> v;
+#pragma unroll
+            for (int e = 0; e < 4; ++e)
+                v[e] = (k < s) ? (t + e <= k ? x[t + e] : T(0)) : (t + e == k ? T(1) : T(0));
+            *reinterpret_cast<Vec4<T>*>(&sL[k][kPadL + t]) = v;
         }
     }
     __syncthreads();
-    T m[kLeaf];
-    inverse_row<T, UNIT>(s, sL, m);
-    T* ib = inv + static_cast<size_t>(b) * kLeaf * kLeaf;
-#pragma unroll
-    for (int j = 0; j < kLeaf; ++j)
-        if (i < s && j < s) ib[j * kLeaf + i] = m[j];
+    rd[k] = UNIT ? T(1) : T(1) / sL[k][kPadL + k];
+    __syncthreads();
+    inverse_from_lds<T>(s, sL, rd, inv + static_cast<size_t>(b) * kLeaf * kLeaf);
 }
 
 template <typename T>
