@@ -129,7 +129,7 @@ _RESTYPE = {
 }
 
 # mifact_precision (include/solve/mixed_refine.h)
-PRECISION = {"fp32": 0, "bf16": 1, "fp16": 2}
+PRECISION = {"fp32": 0, "bf16": 1, "fp16": 2, "fp8": 3}
 # mifact_refine_method
 REFINE_METHOD = {"classical": 0, "gmres": 1, "auto": 2}
 

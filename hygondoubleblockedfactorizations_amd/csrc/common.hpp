@@ -79,6 +79,10 @@ struct mifact_handle_s {
     int trsv_epoch = 0;
     unsigned long long trsv_ticket_base = 0;
 
+    // Low-precision GEMM: packed BF16/FP16/FP8 operand copies and FP8 scales.
+    void* pack_ws = nullptr;
+    size_t pack_ws_bytes = 0;
+
     // Batched path: work-queue order, counter and FP32 staging; the private
     // stream refinement graphs are captured on.
     void* batch_ws = nullptr;

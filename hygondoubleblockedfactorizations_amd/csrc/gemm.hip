@@ -448,13 +448,20 @@ static bool aligned_for_vectors(const void* p, long long ld)
 }
 
 // ------------------------------------------------------------------ low precision
-// C(fp32) += alpha * A * op(B) with A, B read as FP32 and multiplied on
-// v_mfma_f32_32x32x16_{bf16,f16} (FP32 accumulation).  The FP32 tiles are
-// converted while they are staged into LDS, where rows are stored
-// k-contiguous: each lane's MFMA operand (8 consecutive k of one row) is a
-// single 16-byte LDS read.  A lane l of the 32x32x16 operands holds
-// row/column l%32 and k = 8*(l/32) .. +7; the accumulator layout is that of
-// the FP32 32x32 instruction.
+// C(fp32) += alpha * A * op(B) with the products on MFMA in BF16, FP16 or
+// FP8 e4m3, FP32 accumulation.  Both operands are first packed, once per
+// update, into low-precision copies in the handle's pack arena (rows padded
+// with zeros to a multiple of 256, k to a multiple of 128) laid out in
+// slabs: slab s holds 128 bytes of k (64 16-bit or 128 FP8 values) of every
+// row, row after row, so one GEMM stage of a tile is one contiguous block
+// and every lane's MFMA operand is one 16- or 32-byte vector in global
+// memory, LDS and registers; the GEMM streams 2 (or 1) bytes per element
+// instead of 4.  The FP8 copy carries one power-of-two scale per 32 x 32
+// tile (rows x k), stored as an E8M0 byte: the exponent e that brings the
+// tile's largest magnitude into [128, 256) (e4m3 holds up to 448), so that
+// x = q * 2^e with q the e4m3 value; v_mfma_scale_f32_32x32x64_f8f6f4
+// applies 2^(ea + eb) to each lane's 32-element k-block product.
+
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef _Float16 halfx8 __attribute__((ext_vector_type(8)));
 
@@ -474,6 +481,21 @@ template <> struct LowMfma<_Float16> {
     }
 };
 
+struct Fp8Tag {};
+
+template <typename TC>
+struct LowTraits {
+    static constexpr int kBytes = 2;
+};
+template <>
+struct LowTraits<Fp8Tag> {
+    static constexpr int kBytes = 1;
+};
+
+constexpr int kPackRowAlign = 256;   // packed rows: multiple of the largest tile
+constexpr int kPackKAlign = 128;     // packed k: multiple of one 128-byte FP8 stage
+constexpr int kPackTile = 64;        // pack kernel tile (rows x k)
+
 template <typename TC>
 __device__ inline unsigned int pack2(float lo, float hi)
 {
@@ -484,115 +506,161 @@ __device__ inline unsigned int pack2(float lo, float hi)
     return static_cast<unsigned int>(ua) | (static_cast<unsigned int>(ub) << 16);
 }
 
-template <typename TC, int OP, int SHAPE>
-__global__ __launch_bounds__(256) void gemm_lowp_kernel(GemmArgs<float> p)
+// Packs the R x K operand X into slabs (zero padded): element (r, k) at byte
+// (k EB / 128) rows_p 128 + r 128 + (k EB) % 128.  TRANS: X is stored
+// column-major with rows contiguous, X(r, k) = X[r + k ld]; otherwise k is
+// contiguous, X(r, k) = X[k + r ld].  One workgroup per 64 x 64 tile,
+// through LDS.  FP8: the scale of each 32 x 32 sub-tile (one per wave) goes
+// to scale[r / 32][k / 32] (row stride k_p / 32).
+template <typename TC, bool TRANS>
+__global__ __launch_bounds__(256) void pack_kernel(const float* __restrict__ X, long long ld,
+                                                   int R, int K, unsigned char* __restrict__ out,
+                                                   unsigned char* __restrict__ scale, int rows_p,
+                                                   int k_p)
 {
-    using LM = LowMfma<TC>;
-    using v8 = typename LM::v8;
-    constexpr int BM = 128, BN = 128, BK = 32;
-    constexpr int LDK = BK + 8;       // 80-byte rows: 16-byte aligned, conflict-free b128 reads
-    constexpr int FM = 2, FN = 2;     // wave tile 64 x 64 = 2 x 2 blocks of 32 x 32
-    __shared__ __attribute__((aligned(16))) TC sA[2][BM][LDK];
-    __shared__ __attribute__((aligned(16))) TC sB[2][BN][LDK];
+    constexpr int EB = LowTraits<TC>::kBytes;
+    __shared__ float t[kPackTile][kPackTile + 1];
+    const int r0 = blockIdx.x * kPackTile, k0 = blockIdx.y * kPackTile;
+    const int tid = threadIdx.x;
+    for (int e = tid; e < kPackTile * kPackTile; e += 256) {
+        const int rr = TRANS ? (e & 63) : (e >> 6);
+        const int kk = TRANS ? (e >> 6) : (e & 63);
+        const int r = r0 + rr, k = k0 + kk;
+        float v = 0.0f;
+        if (r < R && k < K) v = TRANS ? X[off(r, k, ld)] : X[off(k, r, ld)];
+        t[rr][kk] = v;
+    }
+    __syncthreads();
+    // this tile's 64 k values: one slab (16-bit) or half of one (FP8)
+    const size_t kb = static_cast<size_t>(k0) * EB;
+    unsigned char* row_base = out + (kb / 128) * rows_p * 128 + static_cast<size_t>(r0) * 128 + kb % 128;
+    if constexpr (EB == 2) {
+        for (int c = tid; c < kPackTile * 8; c += 256) {
+            const int rr = c >> 3, kc = (c & 7) * 8;
+            uint4 w;
+            w.x = pack2<TC>(t[rr][kc + 0], t[rr][kc + 1]);
+            w.y = pack2<TC>(t[rr][kc + 2], t[rr][kc + 3]);
+            w.z = pack2<TC>(t[rr][kc + 4], t[rr][kc + 5]);
+            w.w = pack2<TC>(t[rr][kc + 6], t[rr][kc + 7]);
+            *reinterpret_cast<uint4*>(row_base + static_cast<size_t>(rr) * 128 + kc * 2) = w;
+        }
+    } else {
+        __shared__ int sexp[4];
+        const int w = tid >> 6, l = tid & 63;
+        const int sr = (w & 1) * 32, sk = (w >> 1) * 32;
+        float m = 0.0f;
+        for (int e = l; e < 32 * 32; e += 64) m = fmaxf(m, fabsf(t[sr + (e >> 5)][sk + (e & 31)]));
+#pragma unroll
+        for (int o = 32; o > 0; o >>= 1) m = fmaxf(m, __shfl_xor(m, o));
+        if (l == 0) {
+            int ex = 0;
+            if (m > 0.0f && isfinite(m)) ex = ilogbf(m) - 7;   // m * 2^-ex in [128, 256)
+            ex = ex < -127 ? -127 : (ex > 127 ? 127 : ex);
+            sexp[w] = ex;
+            scale[static_cast<size_t>((r0 + sr) >> 5) * (k_p >> 5) + ((k0 + sk) >> 5)] =
+                static_cast<unsigned char>(ex + 127);
+        }
+        __syncthreads();
+        const int rr = tid >> 2, kc = (tid & 3) * 16;
+        const float sc = ldexpf(1.0f, -sexp[(rr >> 5) | ((kc >> 5) << 1)]);
+        int q[4];
+#pragma unroll
+        for (int g = 0; g < 4; ++g) {
+            const float* v = &t[rr][kc + 4 * g];
+            int word = __builtin_amdgcn_cvt_pk_fp8_f32(v[0] * sc, v[1] * sc, 0, false);
+            word = __builtin_amdgcn_cvt_pk_fp8_f32(v[2] * sc, v[3] * sc, word, true);
+            q[g] = word;
+        }
+        *reinterpret_cast<uint4*>(row_base + static_cast<size_t>(rr) * 128 + kc) =
+            make_uint4(q[0], q[1], q[2], q[3]);
+    }
+}
+
+struct PackedGemmArgs {
+    int M, N;
+    int k_p;                       // packed k (multiple of 128)
+    float alpha;
+    const unsigned char* A;        // slabs of rows_a x 128 bytes, rows_a >= tiles_m * BM
+    const unsigned char* B;        // slabs of rows_b x 128 bytes, rows_b >= tiles_n * BN
+    long long slab_a, slab_b;      // bytes per slab (rows x 128)
+    const unsigned char* sa;       // FP8: [rows / 32][k_p / 32] E8M0 scales
+    const unsigned char* sb;
+    float* C;
+    long long ldc;
+    int tiles_m, tiles_n;
+    int cvec;                      // C base and ldc allow float4 access
+};
+
+typedef int intx8 __attribute__((ext_vector_type(8)));
+
+// One stage of a tile: BM (BN) rows x 128 bytes of a slab, contiguous; thread
+// tid copies the 16-byte chunks tid + 256 v.
+template <int ROWS>
+__device__ __attribute__((always_inline)) inline void stage_fetch(const unsigned char* __restrict__ src,
+                                                                  uint4 (&r)[ROWS / 32])
+{
+    const unsigned char* q = src + threadIdx.x * 16;
+#pragma unroll
+    for (int v = 0; v < ROWS / 32; ++v) r[v] = *reinterpret_cast<const uint4*>(q + v * 4096);
+}
+
+// Chunk tid + 256 v is row tid / 8 + 32 v, bytes (tid % 8) * 16 .. +15.
+template <int ROWS, int LDB>
+__device__ __attribute__((always_inline)) inline void stage_store(unsigned char* dst,
+                                                                  const uint4 (&r)[ROWS / 32])
+{
+    unsigned char* q = dst + (threadIdx.x >> 3) * LDB + (threadIdx.x & 7) * 16;
+#pragma unroll
+    for (int v = 0; v < ROWS / 32; ++v) *reinterpret_cast<uint4*>(q + v * 32 * LDB) = r[v];
+}
+
+// FP8: the scale words (4 k-blocks of one stage) of COUNT consecutive
+// 32-row blocks starting at block rb.
+template <int COUNT>
+__device__ __attribute__((always_inline)) inline void scale_fetch(const unsigned char* __restrict__ sc,
+                                                                  int kbs, int rb, int st,
+                                                                  unsigned int (&w)[COUNT])
+{
+#pragma unroll
+    for (int i = 0; i < COUNT; ++i)
+        w[i] = *reinterpret_cast<const unsigned int*>(sc + static_cast<size_t>(rb + i) * kbs + st * 4);
+}
+
+// C(M x N) += alpha * A_p B_p^T on MFMA.  BM x BN tile per workgroup of four
+// waves (2 x 2, each (BM/2) x (BN/2) as 32 x 32 MFMA blocks), 128 bytes of
+// k per stage, LDS double-buffered (row stride 144 bytes).  16-bit: four
+// v_mfma_f32_32x32x16 steps per stage, lane l holding row l % 32 and k =
+// 8 (l / 32) .. +7 of the step; FP8: two v_mfma_scale_f32_32x32x64 steps,
+// lane l (h = l / 32) holding k = 16 h .. +15 and 32 + 16 h .. +15 of the
+// step: the instruction's 32-k scale block b is made of bytes 16 b .. +15
+// of lanes x and x + 32 and takes its scale from lane x + 32 b, so lane
+// half h supplies the scale of k-block h (checked on the GPU with one-hot
+// scales).
+template <typename TC, int BM, int BN, int SHAPE>
+__global__ __launch_bounds__(256) void gemm_packed_kernel(PackedGemmArgs p)
+{
+    constexpr int EB = LowTraits<TC>::kBytes;
+    constexpr int SB = 128;                 // bytes of k per row per stage
+    constexpr int LDB = SB + 16;            // LDS row stride (bytes)
+    constexpr int FM = BM / 64, FN = BN / 64;
+    constexpr int VA = BM / 32, VB = BN / 32;   // 16-byte chunks per thread per stage
+    __shared__ __attribute__((aligned(16))) unsigned char sA[2][BM * LDB];
+    __shared__ __attribute__((aligned(16))) unsigned char sB[2][BN * LDB];
 
     int tm, tn;
     if (SHAPE == kShapeLower) tile_lower(blockIdx.x, p.tiles_m, p.tiles_n, tm, tn);
     else tile_full(blockIdx.x, p.tiles_m, p.tiles_n, tm, tn);
     const int m0 = tm * BM, n0 = tn * BN;
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-    const int wm = wave % 2, wn = wave / 2;
-    const int M = p.M, N = p.N, K = p.K;
-    const float* A = p.A;
-    const float* B = p.B;
-    const bool vec = p.cvec >> 1;     // operands vector-aligned (bit 1)
+    const int wm = wave & 1, wn = wave >> 1;
+    const int nst = p.k_p * EB / SB;                          // stages = slabs
+    const unsigned char* ga = p.A + static_cast<size_t>(m0) * SB;
+    const unsigned char* gb = p.B + static_cast<size_t>(n0) * SB;
+    const int kbs = p.k_p >> 5;
+    const int rba = (m0 + wm * (BM / 2)) >> 5, rbb = (n0 + wn * (BN / 2)) >> 5;
 
-    // staging registers: A and NT-B as (4 rows) x (2 k) pairs, NN-B as (1 col) x (4 k)
-    float4 ra[2][2];
-    float4 rb[4];
-
-    // M-contiguous operand: item f -> rows r0 + 4 (f % 32) .. +3, k = k0 + 2 (f / 32), +1
-    auto load_mk = [&](const float* X, long long ldx, int R, int r0, int k0, float4 (&dst)[2][2]) {
-#pragma unroll
-        for (int v = 0; v < 2; ++v) {
-            const int f = tid + v * 256;
-            const int r = r0 + 4 * (f & 31);
-            const int k = k0 + 2 * (f >> 5);
-#pragma unroll
-            for (int h = 0; h < 2; ++h) {
-                const int kk = k + h;
-                float4 x = make_float4(0.f, 0.f, 0.f, 0.f);
-                if (kk < K) {
-                    if (vec && r + 4 <= R) {
-                        x = *reinterpret_cast<const float4*>(X + off(r, kk, ldx));
-                    } else {
-                        if (r + 0 < R) x.x = X[off(r + 0, kk, ldx)];
-                        if (r + 1 < R) x.y = X[off(r + 1, kk, ldx)];
-                        if (r + 2 < R) x.z = X[off(r + 2, kk, ldx)];
-                        if (r + 3 < R) x.w = X[off(r + 3, kk, ldx)];
-                    }
-                }
-                dst[v][h] = x;
-            }
-        }
-    };
-    auto store_mk = [&](TC (*S)[LDK], const float4 (&src)[2][2]) {
-#pragma unroll
-        for (int v = 0; v < 2; ++v) {
-            const int f = tid + v * 256;
-            const int r = 4 * (f & 31);
-            const int k = 2 * (f >> 5);
-            const float4 a = src[v][0], b = src[v][1];
-            *reinterpret_cast<unsigned int*>(&S[r + 0][k]) = pack2<TC>(a.x, b.x);
-            *reinterpret_cast<unsigned int*>(&S[r + 1][k]) = pack2<TC>(a.y, b.y);
-            *reinterpret_cast<unsigned int*>(&S[r + 2][k]) = pack2<TC>(a.z, b.z);
-            *reinterpret_cast<unsigned int*>(&S[r + 3][k]) = pack2<TC>(a.w, b.w);
-        }
-    };
-    // K-contiguous operand (NN B): item f -> column c0 + f / 8, k = k0 + 4 (f % 8) .. +3
-    auto load_kn = [&](int k0) {
-#pragma unroll
-        for (int v = 0; v < 4; ++v) {
-            const int f = tid + v * 256;
-            const int c = n0 + (f >> 3);
-            const int k = k0 + 4 * (f & 7);
-            float4 x = make_float4(0.f, 0.f, 0.f, 0.f);
-            if (c < N) {
-                if (vec && k + 4 <= K) {
-                    x = *reinterpret_cast<const float4*>(B + off(k, c, p.ldb));
-                } else {
-                    if (k + 0 < K) x.x = B[off(k + 0, c, p.ldb)];
-                    if (k + 1 < K) x.y = B[off(k + 1, c, p.ldb)];
-                    if (k + 2 < K) x.z = B[off(k + 2, c, p.ldb)];
-                    if (k + 3 < K) x.w = B[off(k + 3, c, p.ldb)];
-                }
-            }
-            rb[v] = x;
-        }
-    };
-    auto store_kn = [&](TC (*S)[LDK]) {
-#pragma unroll
-        for (int v = 0; v < 4; ++v) {
-            const int f = tid + v * 256;
-            const int c = f >> 3;
-            const int k = 4 * (f & 7);
-            uint2 w;
-            w.x = pack2<TC>(rb[v].x, rb[v].y);
-            w.y = pack2<TC>(rb[v].z, rb[v].w);
-            *reinterpret_cast<uint2*>(&S[c][k]) = w;
-        }
-    };
-    float4 rbt[2][2];
-    auto fetch = [&](int k0) {
-        load_mk(A, p.lda, M, m0, k0, ra);
-        if (OP == kOpNT) load_mk(B, p.ldb, N, n0, k0, rbt);
-        else load_kn(k0);
-    };
-    auto stage = [&](int buf) {
-        store_mk(sA[buf], ra);
-        if (OP == kOpNT) store_mk(sB[buf], rbt);
-        else store_kn(sB[buf]);
-    };
-
+    uint4 ra[VA], rb[VB];
+    unsigned int rsa[FM], rsb[FN];   // FP8: 4 k-block scales (one stage) per 32-row block
     floatx16 acc[FM][FN];
 #pragma unroll
     for (int i = 0; i < FM; ++i)
@@ -601,47 +669,114 @@ __global__ __launch_bounds__(256) void gemm_lowp_kernel(GemmArgs<float> p)
 #pragma unroll
             for (int r = 0; r < 16; ++r) acc[i][j][r] = 0.f;
 
-    const int nk = (K + BK - 1) / BK;
-    if (nk > 0) {
-        fetch(0);
-        stage(0);
+    stage_fetch<BM>(ga, ra);
+    stage_fetch<BN>(gb, rb);
+    if constexpr (EB == 1) {
+        scale_fetch<FM>(p.sa, kbs, rba, 0, rsa);
+        scale_fetch<FN>(p.sb, kbs, rbb, 0, rsb);
     }
+    stage_store<BM, LDB>(sA[0], ra);
+    stage_store<BN, LDB>(sB[0], rb);
+    unsigned int csa[FM], csb[FN];
+#pragma unroll
+    for (int i = 0; i < FM; ++i) csa[i] = EB == 1 ? rsa[i] : 0u;
+#pragma unroll
+    for (int j = 0; j < FN; ++j) csb[j] = EB == 1 ? rsb[j] : 0u;
     __syncthreads();
-    const int orow = lane & 31;
-    const int okh = (lane >> 5) * 8;
-    for (int kt = 0; kt < nk; ++kt) {
-        const int cur = kt & 1;
-        if (kt + 1 < nk) fetch((kt + 1) * BK);
-#pragma unroll
-        for (int ks = 0; ks < BK / 16; ++ks) {
-            v8 a[FM], b[FN];
-#pragma unroll
-            for (int i = 0; i < FM; ++i)
-                a[i] = *reinterpret_cast<const v8*>(&sA[cur][wm * 64 + i * 32 + orow][ks * 16 + okh]);
-#pragma unroll
-            for (int j = 0; j < FN; ++j)
-                b[j] = *reinterpret_cast<const v8*>(&sB[cur][wn * 64 + j * 32 + orow][ks * 16 + okh]);
-#pragma unroll
-            for (int i = 0; i < FM; ++i)
-#pragma unroll
-                for (int j = 0; j < FN; ++j) acc[i][j] = LM::mma(a[i], b[j], acc[i][j]);
+    const int arow = wm * (BM / 2) + (lane & 31);
+    const int brow = wn * (BN / 2) + (lane & 31);
+    const int khalf = lane >> 5;
+    // The prefetch and the staging run unconditionally (the last stage
+    // re-reads its own slab into the idle buffer): conditional copies would
+    // keep the prefetch registers in memory.
+    for (int st = 0; st < nst; ++st) {
+        const int cur = st & 1;
+        const int nx = st + 1 < nst ? st + 1 : st;
+        stage_fetch<BM>(ga + static_cast<size_t>(nx) * p.slab_a, ra);
+        stage_fetch<BN>(gb + static_cast<size_t>(nx) * p.slab_b, rb);
+        if constexpr (EB == 1) {
+            scale_fetch<FM>(p.sa, kbs, rba, nx, rsa);
+            scale_fetch<FN>(p.sb, kbs, rbb, nx, rsb);
         }
-        if (kt + 1 < nk) stage(cur ^ 1);
+        const unsigned char* la = &sA[cur][arow * LDB];
+        const unsigned char* lb = &sB[cur][brow * LDB];
+        if constexpr (EB == 2) {
+            using LM = LowMfma<TC>;
+            using v8 = typename LM::v8;
+#pragma unroll
+            for (int ks = 0; ks < 4; ++ks) {
+                v8 a[FM], b[FN];
+#pragma unroll
+                for (int i = 0; i < FM; ++i)
+                    a[i] = *reinterpret_cast<const v8*>(la + i * 32 * LDB + ks * 32 + khalf * 16);
+#pragma unroll
+                for (int j = 0; j < FN; ++j)
+                    b[j] = *reinterpret_cast<const v8*>(lb + j * 32 * LDB + ks * 32 + khalf * 16);
+#pragma unroll
+                for (int i = 0; i < FM; ++i)
+#pragma unroll
+                    for (int j = 0; j < FN; ++j) acc[i][j] = LM::mma(a[i], b[j], acc[i][j]);
+            }
+        } else {
+#pragma unroll
+            for (int ks = 0; ks < 2; ++ks) {
+                intx8 a[FM], b[FN];
+                int ea[FM], eb[FN];
+                const int sh = 8 * (ks * 2 + khalf);
+#pragma unroll
+                for (int i = 0; i < FM; ++i) {
+                    const unsigned char* q = la + i * 32 * LDB + ks * 64 + khalf * 16;
+                    const uint4 lo = *reinterpret_cast<const uint4*>(q);
+                    const uint4 hi = *reinterpret_cast<const uint4*>(q + 32);
+                    a[i] = intx8{static_cast<int>(lo.x), static_cast<int>(lo.y),
+                                 static_cast<int>(lo.z), static_cast<int>(lo.w),
+                                 static_cast<int>(hi.x), static_cast<int>(hi.y),
+                                 static_cast<int>(hi.z), static_cast<int>(hi.w)};
+                    ea[i] = static_cast<int>((csa[i] >> sh) & 0xffu);
+                }
+#pragma unroll
+                for (int j = 0; j < FN; ++j) {
+                    const unsigned char* q = lb + j * 32 * LDB + ks * 64 + khalf * 16;
+                    const uint4 lo = *reinterpret_cast<const uint4*>(q);
+                    const uint4 hi = *reinterpret_cast<const uint4*>(q + 32);
+                    b[j] = intx8{static_cast<int>(lo.x), static_cast<int>(lo.y),
+                                 static_cast<int>(lo.z), static_cast<int>(lo.w),
+                                 static_cast<int>(hi.x), static_cast<int>(hi.y),
+                                 static_cast<int>(hi.z), static_cast<int>(hi.w)};
+                    eb[j] = static_cast<int>((csb[j] >> sh) & 0xffu);
+                }
+#pragma unroll
+                for (int i = 0; i < FM; ++i)
+#pragma unroll
+                    for (int j = 0; j < FN; ++j)
+                        acc[i][j] = __builtin_amdgcn_mfma_scale_f32_32x32x64_f8f6f4(
+                            a[i], b[j], acc[i][j], 0, 0, 0, ea[i], 0, eb[j]);
+            }
+        }
+        stage_store<BM, LDB>(sA[cur ^ 1], ra);
+        stage_store<BN, LDB>(sB[cur ^ 1], rb);
+        if constexpr (EB == 1) {
+#pragma unroll
+            for (int i = 0; i < FM; ++i) csa[i] = rsa[i];
+#pragma unroll
+            for (int j = 0; j < FN; ++j) csb[j] = rsb[j];
+        }
         __syncthreads();
     }
 
     const float alpha = p.alpha;
     float* C = p.C;
+    const int M = p.M, N = p.N;
     const bool cvec = p.cvec & 1;
 #pragma unroll
     for (int i = 0; i < FM; ++i) {
 #pragma unroll
         for (int j = 0; j < FN; ++j) {
-            const int col = n0 + wn * 64 + j * 32 + (lane & 31);
+            const int col = n0 + wn * (BN / 2) + j * 32 + (lane & 31);
             if (col >= N) continue;
 #pragma unroll
             for (int g = 0; g < 4; ++g) {
-                const int row = m0 + wm * 64 + i * 32 + g * 8 + (lane >> 5) * 4;
+                const int row = m0 + wm * (BM / 2) + i * 32 + g * 8 + khalf * 4;
                 float* c = C + off(row, col, p.ldc);
                 const bool full = (row + 3 < M) && (SHAPE != kShapeLower || row >= col);
                 if (full && cvec) {
@@ -664,6 +799,115 @@ __global__ __launch_bounds__(256) void gemm_lowp_kernel(GemmArgs<float> p)
     }
 }
 
+namespace {
+
+size_t round_up(size_t v, size_t a) { return (v + a - 1) / a * a; }
+
+// Grows the handle's pack arena to at least `bytes` (synchronizing the
+// stream before freeing the old one).
+bool ensure_pack(mifact_handle h, size_t bytes)
+{
+    if (h->pack_ws != nullptr && h->pack_ws_bytes >= bytes) return true;
+    if (h->pack_ws != nullptr) {
+        if (hipStreamSynchronize(h->stream) != hipSuccess) {
+            report_status(h, mifact_status_hip_error, "gemm_lowp: synchronize before growing");
+            return false;
+        }
+        (void)hipFree(h->pack_ws);
+        h->pack_ws = nullptr;
+        h->pack_ws_bytes = 0;
+    }
+    if (hipMalloc(&h->pack_ws, bytes) != hipSuccess) {
+        h->pack_ws = nullptr;
+        report_status(h, mifact_status_allocation_failed, "gemm_lowp: pack arena");
+        return false;
+    }
+    h->pack_ws_bytes = bytes;
+    return true;
+}
+
+template <typename TC>
+void launch_pack(mifact_handle h, const float* X, long long ld, int R, int K, bool trans,
+                 unsigned char* out, unsigned char* scale, int rows_p, int k_p)
+{
+    const dim3 grid(rows_p / kPackTile, k_p / kPackTile), block(256);
+    if (trans) {
+        hipLaunchKernelGGL((pack_kernel<TC, true>), grid, block, 0, h->stream, X, ld, R, K, out,
+                           scale, rows_p, k_p);
+    } else {
+        hipLaunchKernelGGL((pack_kernel<TC, false>), grid, block, 0, h->stream, X, ld, R, K, out,
+                           scale, rows_p, k_p);
+    }
+}
+
+template <typename TC>
+void gemm_lowp_typed(mifact_handle h, GemmOp op, GemmShape shape, int M, int N, int K,
+                     float alpha, const float* A, long long lda, const float* B, long long ldb,
+                     float* C, long long ldc)
+{
+    constexpr int EB = LowTraits<TC>::kBytes;
+    const int k_p = static_cast<int>(round_up(K, kPackKAlign));
+    const int rows_a = static_cast<int>(round_up(M, kPackRowAlign));
+    // A lower-trapezoid C -= A A(0:N, :)^T reads B as the first N rows of A.
+    const bool share = shape == kShapeLower && op == kOpNT && B == A && ldb == lda;
+    const int rows_b = share ? 0 : static_cast<int>(round_up(N, kPackRowAlign));
+    const size_t bytes_a = static_cast<size_t>(rows_a) * k_p * EB;
+    const size_t bytes_b = static_cast<size_t>(rows_b) * k_p * EB;
+    const size_t sc_a = EB == 1 ? round_up(static_cast<size_t>(rows_a / 32) * (k_p / 32), 256) : 0;
+    const size_t sc_b = EB == 1 ? round_up(static_cast<size_t>(rows_b / 32) * (k_p / 32), 256) : 0;
+    if (!ensure_pack(h, round_up(bytes_a, 256) + round_up(bytes_b, 256) + sc_a + sc_b)) return;
+    unsigned char* base = static_cast<unsigned char*>(h->pack_ws);
+    unsigned char* pa = base;
+    unsigned char* pb = share ? pa : base + round_up(bytes_a, 256);
+    unsigned char* sa = EB == 1 ? base + round_up(bytes_a, 256) + round_up(bytes_b, 256) : nullptr;
+    unsigned char* sb = EB == 1 ? (share ? sa : sa + sc_a) : nullptr;
+
+    const long long slab_a = static_cast<long long>(rows_a) * 128;
+    PackedGemmArgs a{M, N, k_p, alpha, pa, pb, slab_a,
+                     share ? slab_a : static_cast<long long>(rows_b) * 128, sa, sb, C, ldc, 0, 0, 0};
+    a.cvec = aligned_for_vectors<float>(C, ldc) ? 1 : 0;
+    // 256 x 256 tiles where they fill the chip, 128 x 128 otherwise
+    const long long big = static_cast<long long>(ceil_div(M, 256)) * ceil_div(N, 256);
+    const bool large = big >= h->num_cu;
+    const int T = large ? 256 : 128;
+    a.tiles_m = ceil_div(M, T);
+    a.tiles_n = ceil_div(N, T);
+    long long blocks;
+    double mnk_c;
+    if (shape == kShapeLower) {
+        const long long tn = a.tiles_n;
+        blocks = tn * (tn + 1) / 2 + static_cast<long long>(a.tiles_m - a.tiles_n) * tn;
+        mnk_c = static_cast<double>(M) * N - 0.5 * static_cast<double>(N) * (N - 1);
+    } else {
+        blocks = static_cast<long long>(a.tiles_m) * a.tiles_n;
+        mnk_c = static_cast<double>(M) * N;
+    }
+    StageScope scope(h, kStageGemmLowp, 2.0 * mnk_c * K,
+                     4.0 * (static_cast<double>(M) + N) * K + EB * (static_cast<double>(rows_a) + rows_b) * k_p * 2.0
+                         + 8.0 * mnk_c);
+    launch_pack<TC>(h, A, lda, M, K, true, pa, sa, rows_a, k_p);
+    MIFACT_LAUNCHED(h);
+    if (!share) {
+        launch_pack<TC>(h, B, ldb, N, K, op == kOpNT, pb, sb, rows_b, k_p);
+        MIFACT_LAUNCHED(h);
+    }
+    const dim3 grid(static_cast<unsigned>(blocks)), block(256);
+    if (large) {
+        if (shape == kShapeLower)
+            hipLaunchKernelGGL((gemm_packed_kernel<TC, 256, 256, kShapeLower>), grid, block, 0, h->stream, a);
+        else
+            hipLaunchKernelGGL((gemm_packed_kernel<TC, 256, 256, kShapeFull>), grid, block, 0, h->stream, a);
+    } else {
+        if (shape == kShapeLower)
+            hipLaunchKernelGGL((gemm_packed_kernel<TC, 128, 128, kShapeLower>), grid, block, 0, h->stream, a);
+        else
+            hipLaunchKernelGGL((gemm_packed_kernel<TC, 128, 128, kShapeFull>), grid, block, 0, h->stream, a);
+    }
+    MIFACT_LAUNCHED(h);
+}
+
+}  // namespace
+
 void gemm_lowp(mifact_handle h, LowPrecision prec, GemmOp op, GemmShape shape, int M, int N, int K,
                float alpha, const float* A, long long lda, const float* B, long long ldb, float* C,
                long long ldc)
@@ -677,41 +921,20 @@ void gemm_lowp(mifact_handle h, LowPrecision prec, GemmOp op, GemmShape shape, i
         report_status(h, mifact_status_invalid_value, "gemm_lowp: unsupported operand form");
         return;
     }
-    GemmArgs<float> a{M, N, K, alpha, A, lda, B, ldb, C, ldc, 0, 0, 0};
-    const bool vec = aligned_for_vectors<float>(A, lda) && aligned_for_vectors<float>(B, ldb);
-    a.cvec = (aligned_for_vectors<float>(C, ldc) ? 1 : 0) | (vec ? 2 : 0);
-    a.tiles_m = ceil_div(M, 128);
-    a.tiles_n = ceil_div(N, 128);
-    long long blocks;
-    double mnk_c;
-    if (shape == kShapeLower) {
-        const long long tn = a.tiles_n;
-        blocks = tn * (tn + 1) / 2 + static_cast<long long>(a.tiles_m - a.tiles_n) * tn;
-        mnk_c = static_cast<double>(M) * N - 0.5 * static_cast<double>(N) * (N - 1);
-    } else {
-        blocks = static_cast<long long>(a.tiles_m) * a.tiles_n;
-        mnk_c = static_cast<double>(M) * N;
+    switch (prec) {
+    case kPrecBF16:
+        gemm_lowp_typed<__bf16>(h, op, shape, M, N, K, alpha, A, lda, B, ldb, C, ldc);
+        break;
+    case kPrecFP16:
+        gemm_lowp_typed<_Float16>(h, op, shape, M, N, K, alpha, A, lda, B, ldb, C, ldc);
+        break;
+    case kPrecFP8:
+        gemm_lowp_typed<Fp8Tag>(h, op, shape, M, N, K, alpha, A, lda, B, ldb, C, ldc);
+        break;
+    default:
+        report_status(h, mifact_status_invalid_value, "gemm_lowp: precision");
+        break;
     }
-    StageScope scope(h, kStageGemmLowp, 2.0 * mnk_c * K,
-                     4.0 * ((static_cast<double>(M) + N) * K + 2.0 * mnk_c));
-    const dim3 grid(static_cast<unsigned>(blocks)), block(256);
-    auto go = [&](auto tc_tag, auto op_tag, auto shape_tag) {
-        using TC = typename decltype(tc_tag)::type;
-        constexpr int OPV = decltype(op_tag)::value;
-        constexpr int SHV = decltype(shape_tag)::value;
-        hipLaunchKernelGGL((gemm_lowp_kernel<TC, OPV, SHV>), grid, block, 0, h->stream, a);
-    };
-    auto by_shape = [&](auto tc_tag, auto op_tag) {
-        if (shape == kShapeLower) go(tc_tag, op_tag, std::integral_constant<int, kShapeLower>{});
-        else go(tc_tag, op_tag, std::integral_constant<int, kShapeFull>{});
-    };
-    auto by_op = [&](auto tc_tag) {
-        if (op == kOpNT) by_shape(tc_tag, std::integral_constant<int, kOpNT>{});
-        else by_shape(tc_tag, std::integral_constant<int, kOpNN>{});
-    };
-    if (prec == kPrecBF16) by_op(std::type_identity<__bf16>{});
-    else by_op(std::type_identity<_Float16>{});
-    MIFACT_LAUNCHED(h);
 }
 
 template <typename T>

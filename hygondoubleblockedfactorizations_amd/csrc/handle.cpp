@@ -256,6 +256,9 @@ mifact_status mifact_destroy_handle(mifact_handle h)
     if (h->batch_ws != nullptr) {
         (void)hipFree(h->batch_ws);
     }
+    if (h->pack_ws != nullptr) {
+        (void)hipFree(h->pack_ws);
+    }
     if (h->capture_stream != nullptr) {
         (void)hipStreamDestroy(h->capture_stream);
     }

@@ -22,10 +22,12 @@ void gemm(mifact_handle h, GemmOp op, GemmShape shape, int M, int N, int K,
           long long ldc);
 
 // Arithmetic of the trailing updates of the mixed-precision factorizations.
-enum LowPrecision { kPrecFP32 = 0, kPrecBF16 = 1, kPrecFP16 = 2 };
+enum LowPrecision { kPrecFP32 = 0, kPrecBF16 = 1, kPrecFP16 = 2, kPrecFP8 = 3 };
 
-// FP32 C += alpha * A * op(B) with the products on BF16 / FP16 MFMA (FP32
-// accumulation); kPrecFP32 forwards to gemm<float>.  op: kOpNT or kOpNN.
+// FP32 C += alpha * A * op(B) with the products on BF16 / FP16 / FP8-e4m3
+// MFMA (FP32 accumulation; FP8 with a power-of-two scale per 32 x 32 operand
+// tile); kPrecFP32 forwards to gemm<float>.  op: kOpNT or kOpNN.  Uses the
+// handle's pack arena (grown on demand).
 void gemm_lowp(mifact_handle h, LowPrecision prec, GemmOp op, GemmShape shape, int M, int N,
                int K, float alpha, const float* A, long long lda, const float* B, long long ldb,
                float* C, long long ldc);

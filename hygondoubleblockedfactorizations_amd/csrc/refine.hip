@@ -1072,7 +1072,7 @@ mifact_status refine_solve(mifact_handle h, bool sym, int n, int nrhs, const flo
     }
     const long long lda = lda_i, ldf = ldf_i, ldb = ldb_i, ldx = ldx_i;
     const LowPrecision prec = static_cast<LowPrecision>(prm->precision);
-    if (prec != kPrecFP32 && prec != kPrecBF16 && prec != kPrecFP16) {
+    if (prec != kPrecFP32 && prec != kPrecBF16 && prec != kPrecFP16 && prec != kPrecFP8) {
         report_status(h, mifact_status_invalid_value, "refine: unknown precision");
         return mifact_status_invalid_value;
     }
@@ -1208,11 +1208,11 @@ mifact_status refine_solve(mifact_handle h, bool sym, int n, int nrhs, const flo
 
     if (!attempt(prec, true, 0.0f)) return finish(h->last_error);
     if (sym && res->info > 0 && prec != kPrecFP32) {
-        // A BF16/FP16 Cholesky of an ill-conditioned matrix can lose
+        // A BF16/FP16/FP8 Cholesky of an ill-conditioned matrix can lose
         // definiteness; factor the equilibrated matrix plus c u_low I instead
         // (its diagonal lies in [0.5, 2)) and let the refinement remove the
         // shift (Higham & Pranesh, SIAM J. Sci. Comput. 43(1), 2021).
-        const float shift = prec == kPrecBF16 ? 0x1p-6f : 0x1p-9f;
+        const float shift = prec == kPrecBF16 ? 0x1p-6f : (prec == kPrecFP8 ? 0x1p-2f : 0x1p-9f);
         res->diagonal_shift = shift;
         if (!attempt(prec, true, shift)) return finish(h->last_error);
     }

@@ -9,8 +9,8 @@
  * CPU FP64 residual (include/utils/benchmark/cpu_fp64_residual.h:28-201).
  * These entry points add the solve and the precision pattern on top of the
  * same schedule:
- *   - the trailing SYRK/GEMM updates of POTRF/GETRF run on BF16 or FP16 MFMA
- *     with FP32 accumulation; the diagonal POTF2/GETF2, IAMAX, row moves and
+ *   - the trailing SYRK/GEMM updates of POTRF/GETRF run on BF16, FP16 or FP8
+ *     e4m3 (per-tile power-of-two scales) MFMA with FP32 accumulation; the diagonal POTF2/GETF2, IAMAX, row moves and
  *     panel TRSM stay FP32;
  *   - the matrix is equilibrated by powers of two first (exact in FP32), so
  *     the low-precision operands stay in range;
@@ -32,11 +32,16 @@ extern "C" {
 typedef enum mifact_precision {
     mifact_precision_fp32 = 0,  /* FP32 MFMA (the plain path)              */
     mifact_precision_bf16 = 1,  /* BF16 operands, FP32 accumulation        */
-    mifact_precision_fp16 = 2   /* FP16 operands, FP32 accumulation        */
+    mifact_precision_fp16 = 2,  /* FP16 operands, FP32 accumulation        */
+    mifact_precision_fp8 = 3    /* FP8 e4m3 operands with one power-of-two
+                                   scale per 32 x 32 operand tile (E8M0 on
+                                   v_mfma_scale_f32_32x32x64_f8f6f4), FP32
+                                   accumulation                            */
 } mifact_precision;
 
-/* C(m x n) += alpha * A(m x k) * op(B) with the products on BF16/FP16 MFMA
- * and FP32 accumulation.  trans_b = 1: op(B) = B^T, B stored n x k;
+/* C(m x n) += alpha * A(m x k) * op(B) with the products on BF16/FP16/FP8
+ * MFMA and FP32 accumulation (operands packed once per call into the
+ * handle's pack arena, grown on demand).  trans_b = 1: op(B) = B^T, B stored n x k;
  * trans_b = 0: op(B) = B, B stored k x n.  lower = 1 updates only C(i, j)
  * with i >= j (requires m >= n).  FP32 precision uses the FP32 MFMA GEMM. */
 void mifact_gemm_update_lowp(mifact_handle handle, mifact_precision precision, int trans_b,

@@ -47,35 +47,76 @@ def round_bf16(x):
     return b.astype(np.uint32).view(np.float32)
 
 
-def round_lowp(x, prec):
+def round_fp8_tiles(x):
+    """The FP8 pack of a row-by-k operand (rows x k, k the contraction index):
+    per 32 x 32 tile, e = floor(log2 max|x|) - 7 (the tile's largest
+    magnitude scaled into [128, 256)), q = e4m3(x 2^-e) (torch's
+    float8_e4m3fn rounding: to nearest, ties to even), returned as q 2^e."""
+    import torch
+    x = np.asarray(x, np.float64)
+    out = np.zeros_like(x)
+    for r0 in range(0, x.shape[0], 32):
+        for k0 in range(0, x.shape[1], 32):
+            t = x[r0:r0 + 32, k0:k0 + 32]
+            m = np.abs(t).max()
+            e = 0 if m == 0 else int(np.frexp(m)[1]) - 1 - 7
+            q = torch.from_numpy(t * 2.0 ** -e).to(torch.float8_e4m3fn).to(torch.float64).numpy()
+            out[r0:r0 + 32, k0:k0 + 32] = q * 2.0 ** e
+    return out
+
+
+def round_lowp(x, prec, kaxis=1):
+    """x rounded as the GEMM packs it; kaxis: the contraction axis of x."""
+    if prec == "fp8":
+        return round_fp8_tiles(x) if kaxis == 1 else round_fp8_tiles(np.asarray(x).T).T
     return round_bf16(x) if prec == "bf16" else np.asarray(x, np.float32).astype(np.float16).astype(np.float32)
 
 
-PREC = {"fp32": 0, "bf16": 1, "fp16": 2}
+PREC = {"fp32": 0, "bf16": 1, "fp16": 2, "fp8": 3}
 
 
-@pytest.mark.parametrize("prec", ["bf16", "fp16"])
+def tile_pow2(rows, cols, seed):
+    """Powers of two constant on 32 x 32 tiles (2^-2 .. 2^2): with them the
+    FP8 tiles carry different scales (products stay multiples of 2^-4 below
+    2^8, so sums of up to 4096 of them are exact in FP32)."""
+    r = O.rng(seed)
+    e = r.integers(-2, 3, size=((rows + 31) // 32, (cols + 31) // 32))
+    return np.kron(2.0 ** e, np.ones((32, 32)))[:rows, :cols].astype(np.float32)
+
+
+@pytest.mark.parametrize("prec", ["bf16", "fp16", "fp8"])
 @pytest.mark.parametrize("op", ["nt", "nn"])
 @pytest.mark.parametrize("m,n,k,lower", [(256, 256, 64, 0), (300, 130, 77, 0),
-                                         (384, 256, 160, 1), (97, 33, 5, 0)])
+                                         (384, 256, 160, 1), (97, 33, 5, 0),
+                                         (1100, 600, 300, 0), (1300, 1300, 260, 1),
+                                         (4200, 4100, 96, 0), (4352, 4352, 136, 1)])
 def test_gemm_lowp_exact(H, prec, op, m, n, k, lower):
     """Small integers: every product and partial sum is exact in FP32, so the
     MFMA result must equal the integer product bit for bit (pins the
-    32x32x16 operand/accumulator lane layout)."""
+    32x32x16 / scaled 32x32x64 operand and accumulator lane layouts, the
+    packing, both tile sizes, and for FP8 the per-tile scale: the operands
+    are small integers times a power of two per 32 x 32 tile, all exact in
+    e4m3 after the tile's scaling)."""
     log("gemm_lowp exact", prec, op, m, n, k, lower)
     r = O.rng(7)
     lda = m + 3
     A = np.zeros((lda, k), np.float32, order="F")
     A[:m] = r.integers(-4, 5, size=(m, k))
+    if prec == "fp8":
+        A[:m] *= tile_pow2(m, k, 1)
     if op == "nt":
         ldb = n + 1
         B = np.zeros((ldb, k), np.float32, order="F")
         B[:n] = r.integers(-4, 5, size=(n, k))
+        if prec == "fp8":
+            B[:n] *= tile_pow2(n, k, 2)
         prod = A[:m].astype(np.float64) @ B[:n].astype(np.float64).T
     else:
         ldb = k + 2
         B = np.zeros((ldb, n), np.float32, order="F")
         B[:k] = r.integers(-4, 5, size=(k, n))
+        if prec == "fp8":
+            B[:k] *= tile_pow2(n, k, 2).T
         prod = A[:m].astype(np.float64) @ B[:k].astype(np.float64)
     ldc = m + 5
     C = np.zeros((ldc, n), np.float32, order="F")
@@ -96,7 +137,7 @@ def test_gemm_lowp_exact(H, prec, op, m, n, k, lower):
     np.testing.assert_array_equal(out, want.astype(np.float32))
 
 
-@pytest.mark.parametrize("prec", ["bf16", "fp16"])
+@pytest.mark.parametrize("prec", ["bf16", "fp16", "fp8"])
 @pytest.mark.parametrize("op", ["nt", "nn"])
 def test_gemm_lowp_random(H, prec, op):
     m, n, k = 1000, 520, 700
@@ -111,7 +152,7 @@ def test_gemm_lowp_random(H, prec, op):
     sync()
     out = host(dC).astype(np.float64)
     Al = round_lowp(A, prec).astype(np.float64)
-    Bl = round_lowp(B, prec).astype(np.float64)
+    Bl = round_lowp(B, prec, kaxis=1 if op == "nt" else 0).astype(np.float64)
     Bop = Bl.T if op == "nt" else Bl
     want = C - Al @ Bop
     bound = (k + 2) * EPS32 * (np.abs(Al) @ np.abs(Bop) + np.abs(C))
@@ -252,14 +293,15 @@ def refine(H, sym, A, b, prec, nb, bb, tol=0.0, max_it=30, fallback=0, method="c
     ("fp16", 1024, 1, 1e2, "classical"), ("fp16", 1500, 3, 1e3, "classical"),
     ("fp16", 2048, 16, 1e4, "auto"), ("fp16", 1500, 2, 1e6, "auto"),
     ("bf16", 1024, 1, 1e2, "classical"), ("bf16", 2048, 4, 1e4, "auto"),
-    ("bf16", 1024, 1, 1e4, "gmres")])
+    ("bf16", 1024, 1, 1e4, "gmres"), ("fp8", 1024, 1, 1e2, "auto")])
 def test_posv_refine(H, prec, n, nrhs, kappa, method):
     log("posv_refine", prec, n, nrhs, kappa, method)
     A = O.spd_condition_matrix(n, kappa, 21)
     A *= 37.0                                    # exercise the equilibration
     xt = O.rng(22).standard_normal((n, nrhs))
     b = np.asfortranarray(A.astype(np.float64) @ xt)
-    x, res = refine(H, True, A, b, prec, 512, 128, method=method)
+    x, res = refine(H, True, A, b, prec, 512, 128, method=method,
+                    fallback=1 if prec == "fp8" else 0)
     log("  ->", res.as_dict())
     assert res.info == 0 and res.converged == 1
     tol = np.sqrt(n) * np.finfo(np.float64).eps
@@ -277,7 +319,7 @@ def test_posv_refine(H, prec, n, nrhs, kappa, method):
 # with FP32 updates; FP16 updates need GMRES-IR there; BF16-preconditioned
 # FGMRES stalls from n ~ 2048 on, where the FP32 fallback finishes the solve.
 @pytest.mark.parametrize("prec,method", [("fp32", "classical"), ("fp16", "auto"),
-                                         ("bf16", "auto"), ("fp16", "gmres")])
+                                         ("bf16", "auto"), ("fp16", "gmres"), ("fp8", "auto")])
 @pytest.mark.parametrize("n,nrhs", [(1024, 1), (2048, 2), (1111, 17)])
 def test_gesv_refine_uniform(H, prec, method, n, nrhs):
     log("gesv_refine", prec, method, n, nrhs)
@@ -287,7 +329,7 @@ def test_gesv_refine_uniform(H, prec, method, n, nrhs):
     b = np.asfortranarray(A.astype(np.float64) @ xt)
     tol = np.sqrt(n) * np.finfo(np.float64).eps
     x, res = refine(H, False, A, b, prec, 512, 128, tol=tol, method=method,
-                    fallback=1 if prec == "bf16" else 0)
+                    fallback=1 if prec in ("bf16", "fp8") else 0)
     log("  ->", res.as_dict())
     assert res.info == 0 and res.converged == 1
     A64 = A.astype(np.float64)
@@ -368,11 +410,12 @@ def _growth(F, lower_unit):
     return float(np.linalg.norm(np.abs(Lf) @ np.abs(Uf)))
 
 
-@pytest.mark.parametrize("prec", ["bf16", "fp16"])
+@pytest.mark.parametrize("prec", ["bf16", "fp16", "fp8"])
 def test_mixed_factorizations_residual(H, prec):
     """The mixed drivers factor with a backward error set by the update
-    precision.  The trailing updates round both operands to the 16-bit format
-    (unit roundoff u), so each entry of the computed product carries an error
+    precision.  The trailing updates round both operands to the 16-bit or
+    e4m3 format (unit roundoff u = 2^-8, 2^-11, 2^-4; FP8 after the per-tile
+    power-of-two scaling, which is exact), so each entry of the computed product carries an error
     of at most about 2u sum_k |l_ik||u_kj|; the rounding errors are independent
     and add like a random walk, so the normwise residual stays below
     u || |L||U| ||_F / ||A||_F, computed here from the returned factors (the
@@ -380,7 +423,7 @@ def test_mixed_factorizations_residual(H, prec):
     bound is checked for both drivers; a dropped or transposed update term
     gives an O(1) residual, far above it."""
     n = 2048
-    u = 2.0 ** -8 if prec == "bf16" else 2.0 ** -11
+    u = {"bf16": 2.0 ** -8, "fp16": 2.0 ** -11, "fp8": 2.0 ** -4}[prec]
     L = lib()
     A = O.spd_matrix(n, 61)
     dA, info = dev(A), dev_int(1, 7)
