@@ -40,9 +40,13 @@ SOURCES = [
 ]
 
 # the rocblas_handle compatibility layer: its own library (the reference's
-# names, forwarding to libmifact.so), never linked into libmifact.so
+# names, forwarding to libmifact.so), never linked into libmifact.so.  Its
+# file name must not end in "rocblas.so": rocBLAS takes the directory of a
+# loaded module so named as the place of its Tensile kernels, and a process
+# that loaded a library called libmifact_rocblas.so aborted at its first
+# rocBLAS GEMM ("Cannot read .../lib/rocblas/library/TensileLibrary.dat").
 COMPAT_SOURCE = "compat_rocblas.cpp"
-COMPAT_LIB = os.path.join(LIBDIR, "libmifact_rocblas.so")
+COMPAT_LIB = os.path.join(LIBDIR, "libmifact_compat.so")
 
 COMMON_FLAGS = [
     "-O3", "-std=c++20", "-fPIC", f"--offload-arch={ARCH}",
@@ -140,7 +144,7 @@ def build(verbose: bool = True) -> str:
 
 
 def build_compat(verbose: bool = True) -> str:
-    """libmifact_rocblas.so: host-only forwarding code, linked to rocBLAS
+    """libmifact_compat.so: host-only forwarding code, linked to rocBLAS
     (rocblas_get_stream) and libdl."""
     src = os.path.join(CSRC, COMPAT_SOURCE)
     tmp = COMPAT_LIB + ".tmp"

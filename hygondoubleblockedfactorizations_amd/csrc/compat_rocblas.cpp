@@ -1,4 +1,4 @@
-// rocblas_handle compatibility layer (libmifact_rocblas.so): the reference's
+// rocblas_handle compatibility layer (libmifact_compat.so): the reference's
 // exact prototypes (see include/compat/rocblas_compat.h), forwarding to
 // libmifact.so.  Not part of the hot path: per call it reads the rocBLAS
 // handle's stream, binds it to this thread's mifact handle for the current
@@ -18,7 +18,7 @@ constexpr int kMaxDevices = 64;
 
 void die(const char* what, const char* detail)
 {
-    std::fprintf(stderr, "mifact_rocblas: %s%s%s\n", what, detail ? ": " : "", detail ? detail : "");
+    std::fprintf(stderr, "mifact_compat: %s%s%s\n", what, detail ? ": " : "", detail ? detail : "");
     std::exit(EXIT_FAILURE);
 }
 
@@ -28,7 +28,7 @@ void* native_library()
     static void* lib = [] {
         Dl_info info;
         if (dladdr(reinterpret_cast<void*>(&native_library), &info) == 0 || info.dli_fname == nullptr)
-            die("cannot locate libmifact_rocblas.so", nullptr);
+            die("cannot locate libmifact_compat.so", nullptr);
         std::string path(info.dli_fname);
         const size_t slash = path.rfind('/');
         path = (slash == std::string::npos ? std::string(".") : path.substr(0, slash)) + "/libmifact.so";

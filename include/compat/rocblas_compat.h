@@ -7,13 +7,13 @@
  *  kblocked_syrk.h, include/trsm/double_blocked_trsm.h, include/trsm/
  *  right_lower_trsm.h of the reference).
  *
- * Implemented by libmifact_rocblas.so, a thin forwarding library outside
+ * Implemented by libmifact_compat.so, a thin forwarding library outside
  * the hot path: each call reads the rocblas_handle's stream
  * (rocblas_get_stream, the library's only use of rocBLAS), binds it to a
  * per-thread, per-device mifact_handle and forwards to the same-named
  * entry point of libmifact.so, which it loads privately (RTLD_LOCAL), so
  * the two libraries' identically named symbols never meet.  A caller links
- * libmifact_rocblas.so (and rocBLAS) in place of the reference's objects
+ * libmifact_compat.so (and rocBLAS) in place of the reference's objects
  * and keeps its call sites unchanged.  Semantics are those of the native
  * headers (info is LAPACK-global; the tile knobs are accepted and ignored);
  * the StreamPool of the recursive family is accepted and ignored.

@@ -34,7 +34,7 @@ def declared_functions(compat=False):
     return names
 
 
-COMPAT_LIB = os.path.join(os.path.dirname(_lib.LIB_PATH), "libmifact_rocblas.so")
+COMPAT_LIB = os.path.join(os.path.dirname(_lib.LIB_PATH), "libmifact_compat.so")
 
 
 def exported_symbols(path=None):
@@ -101,7 +101,7 @@ def test_missing_library_fails_loudly(tmp_path):
 
 
 def test_compat_library_exports_reference_prototypes():
-    """libmifact_rocblas.so exports every prototype of the compatibility
+    """libmifact_compat.so exports every prototype of the compatibility
     header (the reference's names, taking rocblas_handle); it links rocBLAS
     (for rocblas_get_stream only) while libmifact.so does not, and every
     name it forwards exists in libmifact.so."""
@@ -112,3 +112,11 @@ def test_compat_library_exports_reference_prototypes():
     out = subprocess.run(["ldd", COMPAT_LIB], capture_output=True, text=True).stdout
     assert "rocblas" in out
     assert "libmifact.so" not in out           # loaded privately at run time
+
+
+def test_compat_library_name_not_taken_for_rocblas():
+    """rocBLAS looks for its Tensile kernels beside a loaded module whose
+    file name ends in "rocblas.so"; a compatibility library so named made a
+    later rocBLAS GEMM in the same process abort.  The name must not match."""
+    name = os.path.basename(COMPAT_LIB)
+    assert "rocblas" not in name

@@ -1,6 +1,6 @@
 """GPU parity of the reference-compatible surface: the stand-alone pivot
 launchers (include/getrf/getrf_pivot_kernels.h) and the rocblas_handle
-compatibility library (libmifact_rocblas.so, include/compat/rocblas_compat.h).
+compatibility library (libmifact_compat.so, include/compat/rocblas_compat.h).
 
 Bars: row exchanges are data movement, so results are bit-identical to the
 oracle's LAPACK-laswp restatement (oracle.apply_panel_pivots, which follows
@@ -111,7 +111,7 @@ def compat():
     h = ctypes.c_void_p()
     assert rb.rocblas_create_handle(ctypes.byref(h)) == 0
     assert rb.rocblas_set_stream(h, stream()) == 0
-    cl = ctypes.CDLL(os.path.join(os.path.dirname(_lib.LIB_PATH), "libmifact_rocblas.so"))
+    cl = ctypes.CDLL(os.path.join(os.path.dirname(_lib.LIB_PATH), "libmifact_compat.so"))
     for name in ("double_blocked_potrf", "recursive_potrf"):
         getattr(cl, name).restype = None
     yield rb, h, cl
