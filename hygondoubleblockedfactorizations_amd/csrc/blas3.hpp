@@ -205,25 +205,27 @@ inline bool is_pow2(int v) { return v > 0 && (v & (v - 1)) == 0; }
 // ipiv is written 1-based and global; orig tracks row contents; inv holds
 // kPermLevels snapshot arrays of n ints (level 0 belongs to the caller).
 //
-// Leaves (w <= 32) leave behind the move list of their own exchanges and
-// the inverse of their unit-lower diagonal block, so a node whose left or
-// right child is a leaf needs no snapshot, and its triangular solve needs
-// no separate inversion: a node of width exactly 64 pairs its two leaves'
-// 32 x 32 inverses into the 64 x 64 inverse at pinv + 64 c0 (first leaf on
-// the top-left, second on the bottom-right, inv_pair fills the bottom-left
-// once the block's rows are final).  A left child of power-of-two width
-// >= 64 is made of such 64-nodes, so its solve uses those blocks too; any
-// other width forms its inverses with trtri (the generic trsm_llu).
-// role: 0 lone, 1 first leaf of a 64-node, 2 second leaf of a 64-node.
+// Leaves (w <= LW = kLeafWidth<T>: 64 FP32 columns, 32 FP64) leave behind
+// the move list of their own exchanges and the inverse of their unit-lower
+// diagonal block at pinv + 64 c0, so a node whose left or right child is a
+// leaf needs no snapshot, and its triangular solve needs no separate
+// inversion.  A left child of power-of-two width >= LW is made of leaves at
+// multiples of LW, so its solve uses their blocks too (FP64: a node of width
+// exactly 64 pairs its two 32-leaves' inverses into the 64 x 64 inverse,
+// first leaf on the top-left, second on the bottom-right, inv_pair filling
+// the bottom-left once the block's rows are final); any other width forms
+// its inverses with trtri (the generic trsm_llu).
+// role (FP64 pairs): 0 lone, 1 first leaf of a 64-node, 2 second leaf.
 template <typename T>
 void getrf_panel_rec(mifact_handle h, T* A, long long lda, int n, int c0, int w, int* ipiv,
                      int* orig, int* inv, int nws, int* info, int level, T* pinv, int role = 0)
 {
+    constexpr int LW = kLeafWidth<T>;
     if (w <= 0) return;
     Workspace ws = workspace(h);
     int* mv_src = ws.leaf_moves;
-    int* mv_dst = ws.leaf_moves + 64;
-    if (w <= 32) {
+    int* mv_dst = ws.leaf_moves + 2 * LW;
+    if (w <= LW) {
         T* ib = (role == 2) ? pinv + 64 * static_cast<size_t>(c0 - 32) + 32 + 64 * 32
                             : pinv + 64 * static_cast<size_t>(c0);
         T* iz = (role == 1) ? pinv + 64 * static_cast<size_t>(c0) + 64 * 32 : nullptr;
@@ -234,19 +236,19 @@ void getrf_panel_rec(mifact_handle h, T* A, long long lda, int n, int c0, int w,
         report_status(h, mifact_status_invalid_value, "getrf panel recursion too deep");
         return;
     }
-    const int w1 = split_point(w, 32);
+    const int w1 = split_point(w, LW);
     const int w2 = w - w1;
-    const bool pair = (w == 64);
+    const bool pair = (LW == 32 && w == 64);
     int* inv_l = inv + static_cast<size_t>(level) * nws;
-    if (w1 > 32) perm_snapshot(h, orig, inv_l, c0, n);
+    if (w1 > LW) perm_snapshot(h, orig, inv_l, c0, n);
     getrf_panel_rec<T>(h, A, lda, n, c0, w1, ipiv, orig, inv, nws, info, level + 1, pinv,
                        pair ? 1 : 0);
-    if (w1 > 32) {
+    if (w1 > LW) {
         apply_row_moves<T>(h, A, lda, c0 + w1, c0 + w, c0, w1, ipiv, orig, inv_l, nullptr, 0);
     } else {
         apply_moves<T>(h, A, lda, c0 + w1, c0 + w, 2 * w1, mv_src, mv_dst, nullptr, 0);
     }
-    if (w1 <= 32 || is_pow2(w1)) {
+    if (w1 <= LW || is_pow2(w1)) {
         trsm_llu_inv<T>(h, w1, w2, at(A, lda, c0, c0), lda, at(A, lda, c0, c0 + w1), lda,
                         pinv + 64 * static_cast<size_t>(c0));
     } else {
@@ -254,10 +256,10 @@ void getrf_panel_rec(mifact_handle h, T* A, long long lda, int n, int c0, int w,
     }
     gemm<T>(h, kOpNN, kShapeFull, n - c0 - w1, w2, w1, T(-1), at(A, lda, c0 + w1, c0), lda,
             at(A, lda, c0, c0 + w1), lda, at(A, lda, c0 + w1, c0 + w1), lda);
-    if (w2 > 32) perm_snapshot(h, orig, inv_l, c0 + w1, n);
+    if (w2 > LW) perm_snapshot(h, orig, inv_l, c0 + w1, n);
     getrf_panel_rec<T>(h, A, lda, n, c0 + w1, w2, ipiv, orig, inv, nws, info, level + 1, pinv,
                        pair ? 2 : 0);
-    if (w2 > 32) {
+    if (w2 > LW) {
         apply_row_moves<T>(h, A, lda, c0, c0 + w1, c0 + w1, w2, ipiv, orig, inv_l, nullptr, 0);
     } else {
         apply_moves<T>(h, A, lda, c0, c0 + w1, 2 * w2, mv_src, mv_dst, nullptr, 0);

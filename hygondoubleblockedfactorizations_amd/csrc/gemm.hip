@@ -3,7 +3,9 @@
 //   C(M x N) += alpha * A(M x K) * op(B),   column-major, 64-bit offsets.
 //
 // FP32 runs on v_mfma_f32_32x32x2_f32, FP64 on v_mfma_f64_16x16x4_f64.  A
-// 256-thread workgroup (four wave64s) owns a BM x BN tile of C; each wave
+// workgroup of WAVES_M x WAVES_N wave64s (four, or eight for the 256 x 256
+// tile, whose 64 x 128 wave tiles keep their 128 accumulators in VGPRs at
+// two waves per SIMD) owns a BM x BN tile of C; each wave
 // owns a (BM/WAVES_M) x (BN/WAVES_N) sub-tile made of MFMA blocks whose
 // accumulators stay in registers for the whole K loop.  K advances in BK
 // slices staged global -> registers -> LDS with double buffering (one
@@ -126,8 +128,9 @@ __device__ inline void tile_lower(int bid, int tiles_m, int tiles_n, int& tm, in
 
 template <typename T, int BM, int BN, int BK, int WAVES_M, int WAVES_N, int OP,
           int SHAPE, bool VEC, bool BETA0>
-__global__ __launch_bounds__(256) void gemm_kernel(GemmArgs<T> p)
+__global__ __launch_bounds__(64 * WAVES_M * WAVES_N) void gemm_kernel(GemmArgs<T> p)
 {
+    constexpr int NTHR = 64 * WAVES_M * WAVES_N;
     using MF = Mfma<T>;
     using acc_t = typename MF::acc_t;
     using vec_t = typename Vec<T>::type;
@@ -140,10 +143,11 @@ __global__ __launch_bounds__(256) void gemm_kernel(GemmArgs<T> p)
     constexpr int PAD = (sizeof(T) == 4) ? 32 : 8;
     constexpr int SA = BM + PAD;
     constexpr int SB = BN + PAD;
-    constexpr int A_VECS = BM * BK / VW / 256;  // vectors per thread, A tile
-    constexpr int B_VECS = BN * BK / VW / 256;
-    static_assert(A_VECS >= 1 && B_VECS >= 1, "tile too small for 256 threads");
-    static_assert(WAVES_M * WAVES_N == 4, "four waves per workgroup");
+    constexpr int A_VECS = BM * BK / VW / NTHR;  // vectors per thread, A tile
+    constexpr int B_VECS = BN * BK / VW / NTHR;
+    static_assert(A_VECS >= 1 && B_VECS >= 1, "tile too small for the workgroup");
+    static_assert(A_VECS * VW * NTHR == BM * BK && B_VECS * VW * NTHR == BN * BK,
+                  "tile not a whole number of vectors per thread");
 
     __shared__ T sA[2][BK][SA];
     __shared__ T sB[2][BK][SB];
@@ -177,7 +181,7 @@ __global__ __launch_bounds__(256) void gemm_kernel(GemmArgs<T> p)
         constexpr bool CHECK = decltype(check_tag)::value;
 #pragma unroll
         for (int v = 0; v < A_VECS; ++v) {
-            const int f = tid + v * 256;
+            const int f = tid + v * NTHR;
             T tmp[VW];
             if (OP != kOpTN) {
                 const int kk = f / (BM / VW);
@@ -214,7 +218,7 @@ __global__ __launch_bounds__(256) void gemm_kernel(GemmArgs<T> p)
         }
 #pragma unroll
         for (int v = 0; v < B_VECS; ++v) {
-            const int f = tid + v * 256;
+            const int f = tid + v * NTHR;
             T tmp[VW];
             if (OP == kOpNT) {
                 const int kk = f / (BN / VW);
@@ -254,7 +258,7 @@ __global__ __launch_bounds__(256) void gemm_kernel(GemmArgs<T> p)
     auto store_slice = [&](int buf) {
 #pragma unroll
         for (int v = 0; v < A_VECS; ++v) {
-            const int f = tid + v * 256;
+            const int f = tid + v * NTHR;
             if (OP != kOpTN) {
                 const int kk = f / (BM / VW);
                 const int ii = (f % (BM / VW)) * VW;
@@ -268,7 +272,7 @@ __global__ __launch_bounds__(256) void gemm_kernel(GemmArgs<T> p)
         }
 #pragma unroll
         for (int v = 0; v < B_VECS; ++v) {
-            const int f = tid + v * 256;
+            const int f = tid + v * NTHR;
             if (OP == kOpNT) {
                 const int kk = f / (BN / VW);
                 const int jj = (f % (BN / VW)) * VW;
@@ -406,7 +410,7 @@ static void launch_cfg(mifact_handle h, GemmOp op, GemmShape shape, bool vec, bo
     StageScope scope(h, stage, 2.0 * mnk_c * a.K,
                      sizeof(T) * ((static_cast<double>(a.M) + a.N) * a.K + 2.0 * mnk_c));
     const dim3 grid(static_cast<unsigned>(blocks));
-    const dim3 block(256);
+    const dim3 block(64 * WAVES_M * WAVES_N);
     auto go = [&](auto op_tag, auto shape_tag, auto vec_tag, auto beta_tag) {
         constexpr int OPV = decltype(op_tag)::value;
         constexpr int SHV = decltype(shape_tag)::value;
@@ -952,17 +956,26 @@ static void dispatch(mifact_handle h, GemmOp op, GemmShape shape, bool beta0, in
     const bool vec = aligned_for_vectors<T>(A, lda) && aligned_for_vectors<T>(B, ldb);
     a.cvec = aligned_for_vectors<T>(C, ldc) ? 1 : 0;
     if constexpr (sizeof(T) == 4) {
-        // Large tiles only where they fill the chip; the overwrite (triangular
-        // solve leaf) form always uses 64 x 64 so that one tile spans s <= 64.
-        const long long big = static_cast<long long>(ceil_div(M, 128)) * ceil_div(N, 128);
-        const bool large = !beta0 && N > 64 && M > 64 && big >= 2LL * h->num_cu;
+        // Large tiles only where they give every CU several of them: 256 x 256
+        // (eight waves) for full updates from four tiles per CU, 128 x 128
+        // from two; 64 x 64 otherwise.  The overwrite (triangular-solve leaf)
+        // form always uses 64 x 64 so that one tile spans s <= 64.
+        auto tiles = [&](int t) {
+            const long long tm = ceil_div(M, t), tn = ceil_div(N, t);
+            return shape == kShapeLower ? tn * (tn + 1) / 2 + (tm - tn) * tn : tm * tn;
+        };
         int cfg = h->gemm_config;
-        if (beta0 || (cfg == 0 && !large)) cfg = 3;
-        if (cfg == 0) cfg = 1;
+        if (beta0) {
+            cfg = 3;
+        } else if (cfg == 0) {
+            if (shape == kShapeFull && tiles(256) >= 4LL * h->num_cu) cfg = 4;
+            else if (M > 64 && N > 64 && tiles(128) >= 2LL * h->num_cu) cfg = 1;
+            else cfg = 3;
+        }
         switch (cfg) {
         case 2: launch_cfg<T, 128, 128, 32, 2, 2>(h, op, shape, vec, beta0, a); break;
         case 3: launch_cfg<T, 64, 64, 16, 2, 2>(h, op, shape, vec, beta0, a); break;
-        case 4: launch_cfg<T, 256, 256, 16, 2, 2>(h, op, shape, vec, beta0, a); break;
+        case 4: launch_cfg<T, 256, 256, 16, 4, 2>(h, op, shape, vec, beta0, a); break;
         default: launch_cfg<T, 128, 128, 16, 2, 2>(h, op, shape, vec, beta0, a); break;
         }
     } else {

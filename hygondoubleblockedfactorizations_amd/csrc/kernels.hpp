@@ -59,13 +59,20 @@ template <typename T>
 void trtri_blocks(mifact_handle h, int n, const T* L, long long ldl, T* inv, bool unit,
                   bool upper = false);
 
+// Width of the GETF2 leaves: 64 columns in FP32, 32 in FP64 (the rows of a
+// leaf stay in registers: 2 x 64 FP32 or 2 x 32 FP64 values per thread).
+template <typename T>
+inline constexpr int kLeafWidth = sizeof(T) == 4 ? 64 : 32;
+
 // Cooperative partial-pivot LU of a leaf panel: rows [c0, n) x columns
-// [c0, c0 + w) of the full matrix (w <= 32).  Writes ipiv[c0 + j] (global,
-// 1-based), carries orig[] (the origin of each row's content) along with the
-// row exchanges and records the first zero pivot in *info.  Optionally:
+// [c0, c0 + w) of the full matrix (w <= kLeafWidth<T>).  Writes ipiv[c0 + j]
+// (global, 1-based), carries orig[] (the origin of each row's content) along
+// with the row exchanges and records the first zero pivot in *info.
+// Optionally:
 //   inv      receives the inverse of the leaf's unit-lower w x w diagonal
-//            block (32 x 32, ld 64, identity rows past w); inv_zero, when
-//            set, names a 32 x 32 block (ld 64) that is cleared;
+//            block (kLeafWidth<T> square, ld 64, identity rows past w);
+//            inv_zero, when set, names a 32 x 32 block (ld 64) that is
+//            cleared (FP64: the off-diagonal block of a 64-column pair);
 //   mv_src,  receive the move list of the leaf's own exchanges (2w entries,
 //   mv_dst   global rows, src -1 for "no move"), the same list
 //            apply_row_moves would derive from a snapshot taken before it.

@@ -32,8 +32,7 @@
 
 namespace mifact {
 
-constexpr int kLeafW = 32;
-constexpr int kMaxG = 128;           // workgroups of a leaf (rows <= 128 * 512 * 4)
+constexpr int kMaxG = 128;           // workgroups of a leaf (rows <= 128 * 512)
 constexpr int kIdxNone = 0x7fffffff; // "no candidate" row index
 
 template <typename T> __device__ inline T sfmin();
@@ -159,17 +158,18 @@ __device__ inline void wave_argmax_fast(double& v, int& i)
 // coherent 64-bit store, so a reader that sees the column's epoch in a word
 // also sees that word's payload; readers poll until every word carries the
 // epoch (no counter, no fence).  Workgroup g publishes record g: its pivot
-// candidate (row, |value|), the candidate's whole row and its origin tag.
-// Every workgroup gathers all G records in one round trip, so the pivot row
-// is at hand as soon as the pivot is known.  A workgroup can only overwrite
-// a parity slot (column j + 2) after it has gathered every record of column
-// j + 1, which each workgroup publishes only after it has finished with
-// column j.
+// candidate (row, |value|), the candidate's whole row (column order) and
+// its origin tag.  Every workgroup gathers all G records in one round trip,
+// so the pivot row is at hand as soon as the pivot is known.  A workgroup
+// can only overwrite a parity slot (column j + 2) after it has gathered
+// every record of column j + 1, which each workgroup publishes only after
+// it has finished with column j.
 template <typename T>
 struct Rec {
+    static constexpr int LW = kLeafWidth<T>;
     static constexpr int PW = sizeof(T) / 4;   // payload words per T
-    static constexpr int ROW = 0;               // kLeafW values
-    static constexpr int TAG = PW * kLeafW;     // the row's origin tag
+    static constexpr int ROW = 0;               // LW values
+    static constexpr int TAG = PW * LW;         // the row's origin tag
     static constexpr int VAL = TAG + 1;         // |value| of the candidate
     static constexpr int IDX = VAL + PW;        // candidate row (label)
     static constexpr int RW = IDX + 1;          // words per record
@@ -177,6 +177,9 @@ struct Rec {
 };
 
 constexpr int kLeafThreads = 512;    // 8 waves: rows of the panel in registers
+constexpr int kLeafWaves = kLeafThreads / 64;
+constexpr int kLeafQ = 16;           // columns per phase: the live window shrinks by kLeafQ
+constexpr int kLeafU = 4;            // columns per group: registers rotate by kLeafU
 
 template <typename T>
 struct LeafArgs {
@@ -213,12 +216,11 @@ __device__ inline unsigned long long xload64(const unsigned long long* p)
                              __HIP_MEMORY_SCOPE_AGENT);
 }
 
-__device__ inline void to_words(float v, unsigned int* w) { w[0] = __float_as_uint(v); }
-__device__ inline void to_words(double v, unsigned int* w)
+__device__ inline unsigned int word_of(float v, int) { return __float_as_uint(v); }
+__device__ inline unsigned int word_of(double v, int half)
 {
     const unsigned long long b = static_cast<unsigned long long>(__double_as_longlong(v));
-    w[0] = static_cast<unsigned int>(b);
-    w[1] = static_cast<unsigned int>(b >> 32);
+    return static_cast<unsigned int>(half ? (b >> 32) : b);
 }
 template <typename T> __device__ inline T from_words(const unsigned int* w);
 template <> __device__ inline float from_words<float>(const unsigned int* w)
@@ -231,82 +233,98 @@ template <> __device__ inline double from_words<double>(const unsigned int* w)
     return __longlong_as_double(static_cast<long long>(b));
 }
 
+template <typename T>
+using LVec4 = T __attribute__((ext_vector_type(4)));
+
 // Shared state of one leaf workgroup.
 template <typename T>
 struct LeafShared {
-    T wv[kLeafThreads / 64];
-    int wi[kLeafThreads / 64];
-    unsigned int stage[kLeafThreads / 64][Rec<T>::RW];  // each wave's candidate row
+    static constexpr int LW = kLeafWidth<T>;
+    // each wave's candidate row in register order (window, then the columns
+    // finished in earlier phases), its multiplier of the previous column
+    // and its origin tag
+    alignas(16) T stage[kLeafWaves][LW];
+    // the previous column's pivot row in the register order of the current
+    // step, zero outside the columns that still take its update
+    alignas(16) T urow[2][LW];
+    T stage_l[kLeafWaves];
+    unsigned int stage_tag[kLeafWaves];
+    T wv[kLeafWaves];
+    int wi[kLeafWaves];
     unsigned int rec[kMaxG][Rec<T>::RW];  // payloads of every record of the column
     int abort;                            // a bounded wait expired
-    int piv[kLeafW];                      // local pivot row of each column (workgroup 0)
-    T lb[kLeafW][kLeafW + 1];             // workgroup 0: the leaf's first kLeafW rows
+    int piv[LW];                          // local pivot row of each column (workgroup 0)
+    T lb[LW][LW + 1];                     // workgroup 0: the leaf's pivot rows
 };
 
-// Register rotation: the leaf's columns are processed in groups of kLeafU
-// with register indices 0 .. kLeafU - 1, then every row rotates left by
-// kLeafU, so the column loop is a runtime loop over fixed register indices.
-// (Groups of 2 halve the rotation moves; 4 would exceed the register budget
-// of 4 rows per thread.)
-constexpr int kLeafU = 2;
-
-template <typename T, int RPT>
-__device__ inline void rotate_rows(T (&x)[RPT][kLeafW])
+// Register index, at a phase whose live window is W wide, of column c < LW - W
+// (finished in an earlier phase): each phase leaves its kLeafQ columns, in
+// order, just above the next window, so the blocks of earlier phases lie
+// above in reverse phase order.
+template <int LW>
+__device__ inline int leaf_index(int W, int c)
 {
-#pragma unroll
-    for (int r = 0; r < RPT; ++r) {
-        T t[kLeafU];
-#pragma unroll
-        for (int u = 0; u < kLeafU; ++u) t[u] = x[r][u];
-#pragma unroll
-        for (int c = 0; c < kLeafW - kLeafU; ++c) x[r][c] = x[r][c + kLeafU];
-#pragma unroll
-        for (int u = 0; u < kLeafU; ++u) x[r][kLeafW - kLeafU + u] = t[u];
-    }
+    const int done = (LW - W) / kLeafQ;   // phases finished
+    return W + kLeafQ * (done - 1 - c / kLeafQ) + c % kLeafQ;
 }
 
-// Column j = j0 + U of a leaf (register index U); see getf2_leaf_kernel.
-// Returns false when a bounded wait expired (uniform over the workgroup).
-template <typename T, int RPT, int NP, int U>
-__device__ __attribute__((always_inline)) inline bool leaf_column(
-    const LeafArgs<T>& a, LeafShared<T>& sh, T (&x)[RPT][kLeafW], int (&tag)[RPT],
-    int (&lab)[RPT], int j0, int g)
+// Column j = base + U of a leaf (see getf2_leaf_kernel).  Register index i
+// of x holds column base + i for i < LW - base, the columns finished in this
+// phase (wrapped) above that up to W, and above W the columns of earlier
+// phases (leaf_index).
+// lp carries the multipliers of column j - 1, whose update of the columns
+// after j is deferred to this step so that it runs while the column's
+// records are in flight.  Returns false when a bounded wait expired
+// (uniform over the workgroup).
+template <typename T, int RPT, int NP, int W, int U>
+__device__ __attribute__((always_inline)) inline bool leaf_step(
+    const LeafArgs<T>& a, LeafShared<T>& sh, T (&x)[RPT][kLeafWidth<T>], int (&tag)[RPT],
+    int (&lab)[RPT], T (&lp)[RPT], int base, int g)
 {
     using R = Rec<T>;
-    constexpr int NW = kLeafThreads / 64;
-    const int j = j0 + U;
+    constexpr int LW = kLeafWidth<T>;
+    constexpr int QP = LW - W;     // columns finished in earlier phases
+    const int j = base + U;
     const int tid = threadIdx.x;
+    const int lane = tid & 63;
+    const int wave = tid >> 6;
     const int G = a.G;
     const unsigned int epoch = a.epoch0 + static_cast<unsigned int>(j);
     unsigned long long* slot = a.xrec + static_cast<size_t>(j & 1) * R::PARITY;
+    const int par = j & 1;
+    const bool has_prev = j > 0;
 
-    // 1. local candidate; each wave's winning lane stages its row in LDS
+    // 1. local candidate of column j (current in x[r][U]); each wave's
+    //    winning lane stages its row
     T best = T(-1);
     int bi = kIdxNone;
 #pragma unroll
     for (int r = 0; r < RPT; ++r)
         if (lab[r] >= j && lab[r] != kIdxNone) amax_pick(best, bi, fabs(x[r][U]), lab[r]);
     wave_argmax_fast(best, bi);
-    const int wave = tid >> 6;
 #pragma unroll
     for (int r = 0; r < RPT; ++r) {
         if (bi != kIdxNone && lab[r] == bi) {
-            unsigned int* st = sh.stage[wave];
 #pragma unroll
-            for (int c = 0; c < kLeafW; ++c) to_words(x[r][c], st + R::PW * c);
-            st[R::TAG] = static_cast<unsigned int>(tag[r]);
+            for (int q = 0; q < LW; q += 4)
+                *reinterpret_cast<LVec4<T>*>(&sh.stage[wave][q]) =
+                    LVec4<T>{x[r][q], x[r][q + 1], x[r][q + 2], x[r][q + 3]};
+            sh.stage_l[wave] = lp[r];
+            sh.stage_tag[wave] = static_cast<unsigned int>(tag[r]);
         }
     }
-    if ((tid & 63) == 0) {
+    if (lane == 0) {
         sh.wv[wave] = best;
         sh.wi[wave] = bi;
     }
     __syncthreads();
+
+    // 2. the workgroup's candidate: best over the waves
     T wb = sh.wv[0];
     int wi = sh.wi[0];
     int qw = 0;
 #pragma unroll
-    for (int q = 1; q < NW; ++q) {
+    for (int q = 1; q < kLeafWaves; ++q) {
         const T ov = sh.wv[q];
         const int oi = sh.wi[q];
         if (ov > wb || (ov == wb && oi < wi)) {
@@ -316,22 +334,53 @@ __device__ __attribute__((always_inline)) inline bool leaf_column(
         }
     }
 
-    // 2. publish the record, one word per thread
+    // 3. publish the record, one word per thread; the row's columns after j
+    //    take the deferred update of column j - 1 here (the same fused
+    //    multiply-add its owner applies below)
     if (tid < R::RW) {
         unsigned int word;
-        if (tid < R::VAL) {
-            word = (wi == kIdxNone) ? (tid == R::TAG ? 0xffffffffu : 0u) : sh.stage[qw][tid];
+        if (wi == kIdxNone) {
+            word = (tid == R::TAG) ? 0xffffffffu
+                 : (tid >= R::VAL && tid < R::IDX) ? word_of(T(-1), tid - R::VAL)
+                 : (tid == R::IDX) ? static_cast<unsigned int>(kIdxNone) : 0u;
+        } else if (tid < R::TAG) {
+            const int c = tid / R::PW;
+            const int i = (c < QP) ? leaf_index<LW>(W, c) : ((c >= base) ? c - base : c - base + W);
+            T v = sh.stage[qw][i];
+            if (has_prev && i < W) v = fma(-sh.stage_l[qw], sh.urow[par ^ 1][i], v);
+            word = word_of(v, tid % R::PW);
+        } else if (tid == R::TAG) {
+            word = sh.stage_tag[qw];
         } else if (tid < R::IDX) {
-            unsigned int vw[R::PW];
-            to_words(wi == kIdxNone ? T(-1) : wb, vw);
-            word = vw[tid - R::VAL];
+            word = word_of(wb, tid - R::VAL);
         } else {
             word = static_cast<unsigned int>(wi);
         }
         xstore64(slot + static_cast<size_t>(g) * R::RW + tid, word, epoch);
     }
 
-    // 3. gather: each thread polls its own words of the G records until all
+    // 4. deferred update of column j - 1 on own rows, columns after j
+    if (has_prev) {
+#pragma unroll
+        for (int q = 0; q < W; q += 4) {
+            if (q + 3 <= U) continue;
+            const LVec4<T> u4 = *reinterpret_cast<const LVec4<T>*>(&sh.urow[par ^ 1][q]);
+#pragma unroll
+            for (int e = 0; e < 4; ++e) {
+                if (q + e <= U) continue;
+#pragma unroll
+                for (int r = 0; r < RPT; ++r) {
+                    x[r][q + e] = fma(-lp[r], u4[e], x[r][q + e]);
+                    // pinned here: not sunk past the gather below (the point
+                    // is to run while the records travel), and the
+                    // pivot-row reads are not all held at once
+                    asm volatile("" : "+v"(x[r][q + e]));
+                }
+            }
+        }
+    }
+
+    // 5. gather: each thread polls its own words of the G records until all
     //    carry the column's epoch; one barrier joins the workgroup
     {
         const int total = G * R::RW;
@@ -369,13 +418,13 @@ __device__ __attribute__((always_inline)) inline bool leaf_column(
         return false;
     }
 
-    // 4. global pivot, computed by every wave from the gathered records (lane
-    //    t takes record t, then the wave arg-max; no barrier): max over the G
-    //    candidates, smallest label on ties (the key carries the record index
-    //    in its low 8 bits)
+    // 6. global pivot, computed by every wave from the gathered records (lane
+    //    t takes record t, then the wave arg-max): max over the G candidates,
+    //    smallest label on ties (the key carries the record index in its low
+    //    8 bits)
     T gb = T(-1);
     int gk = kIdxNone;
-    for (int t = tid & 63; t < G; t += 64) {
+    for (int t = lane; t < G; t += 64) {
         const int idx = static_cast<int>(sh.rec[t][R::IDX]);
         amax_pick(gb, gk, from_words<T>(&sh.rec[t][R::VAL]),
                   idx == kIdxNone ? kIdxNone : (idx << 8) | t);
@@ -384,50 +433,53 @@ __device__ __attribute__((always_inline)) inline bool leaf_column(
     const bool none = gk == kIdxNone;
     const int p = none ? j : (gk >> 8);
     const unsigned int* prow = sh.rec[none ? 0 : (gk & 0xff)];
-    const T piv = none ? T(0) : from_words<T>(prow + R::ROW + R::PW * U);
+    const T piv = none ? T(0) : from_words<T>(prow + R::ROW + R::PW * j);
+    T l[RPT];
+#pragma unroll
+    for (int r = 0; r < RPT; ++r) l[r] = T(0);
     if (piv != T(0)) {
         const bool recip = fabs(piv) >= sfmin<T>();
         const T rcp = T(1) / piv;
-        T l[RPT];
 #pragma unroll
         for (int r = 0; r < RPT; ++r) {
             if (lab[r] == p) lab[r] = j;
             else if (lab[r] == j) lab[r] = p;
         }
-        if (recip) {
-#pragma unroll
-            for (int r = 0; r < RPT; ++r) l[r] = x[r][U] * rcp;
-        } else {
-#pragma unroll
-            for (int r = 0; r < RPT; ++r) l[r] = x[r][U] / piv;
-        }
 #pragma unroll
         for (int r = 0; r < RPT; ++r) {
             const bool upd = lab[r] > j && lab[r] != kIdxNone;
-            l[r] = upd ? l[r] : T(0);
-            if (upd) x[r][U] = l[r];
-        }
-        // register index c holds column j0 + c; indices from kLeafW - j0 on
-        // hold finished columns (wrapped) and take no update (every load is
-        // issued, the finished ones are then zeroed)
-        T u[kLeafW];
-#pragma unroll
-        for (int c = U + 1; c < kLeafW; ++c) u[c] = from_words<T>(prow + R::ROW + R::PW * c);
-#pragma unroll
-        for (int c = U + 1; c < kLeafW; ++c) {
-            const T uc = (c < kLeafW - j0) ? u[c] : T(0);
-#pragma unroll
-            for (int r = 0; r < RPT; ++r) x[r][c] -= l[r] * uc;
+            const T lv = recip ? x[r][U] * rcp : x[r][U] / piv;
+            l[r] = upd ? lv : T(0);
+            if (upd) x[r][U] = lv;
         }
     } else if (g == 0 && tid == 0) {
         atomicCAS(a.info, 0, a.c0 + j + 1);
     }
-    // workgroup 0 keeps row j of the factors (final now: its multipliers are
-    // set and it takes no further update), in column order
-    if (g == 0 && tid < kLeafW) {
-        const int rc = (tid - j0 + kLeafW) % kLeafW;
-        sh.lb[j][tid] = none ? T(0) : from_words<T>(prow + R::ROW + R::PW * rc);
+
+    // 7. column j + 1 takes this pivot's update now (its candidates come
+    //    next); the pivot row goes to urow in the register order of step
+    //    j + 1 for the deferred update of the other columns
+    {
+        const T un = (piv != T(0) && j + 1 < LW) ? from_words<T>(prow + R::ROW + R::PW * (j + 1))
+                                                  : T(0);
+#pragma unroll
+        for (int r = 0; r < RPT; ++r) x[r][U + 1] = fma(-l[r], un, x[r][U + 1]);
     }
+    if (tid < LW) {
+        constexpr int UN = (U == kLeafU - 1) ? 0 : U + 1;
+        const int base_n = (U == kLeafU - 1) ? base + kLeafU : base;
+        const int i = tid;
+        T v = T(0);
+        if (piv != T(0) && i > UN && i < LW - base_n)
+            v = from_words<T>(prow + R::ROW + R::PW * (base_n + i));
+        sh.urow[par][i] = v;
+    }
+#pragma unroll
+    for (int r = 0; r < RPT; ++r) lp[r] = l[r];
+
+    // workgroup 0 keeps the pivot row (its multipliers before column j are
+    // final) and the pivot
+    if (g == 0 && tid < LW) sh.lb[j][tid] = none ? T(0) : from_words<T>(prow + R::ROW + R::PW * tid);
     if (g == 0 && tid == 0) {
         a.ipiv[a.c0 + j] = a.c0 + p + 1;
         sh.piv[j] = p;
@@ -435,28 +487,75 @@ __device__ __attribute__((always_inline)) inline bool leaf_column(
     return true;
 }
 
-// Cooperative partial-pivot LU of an m x w leaf panel (w <= 32).  Blocks
-// b with b % stride == 0 hold rows: workgroup g = b / stride loads rows
-// [g * 512 * RPT, (g + 1) * 512 * RPT) of the panel, RPT per thread, into
-// registers for the whole leaf; the other blocks exit at once.  The
-// hardware deals consecutive blocks to the XCDs in turn, so a stride of 8
-// keeps the exchanging workgroups on one XCD.  Rows never move between
-// threads: each register row carries a label, its current row index in the
-// LAPACK sense, and a row exchange swaps two labels.  At the end every row
-// is stored at its label.  Per column j:
-//   1. local candidate: wave64 DPP arg-max, then across the 8 waves in LDS
+// Rotates the live window x[r][0, W) left by kLeafU.
+template <typename T, int RPT, int W>
+__device__ __attribute__((always_inline)) inline void leaf_rotate(T (&x)[RPT][kLeafWidth<T>])
+{
+#pragma unroll
+    for (int r = 0; r < RPT; ++r) {
+        T t[kLeafU];
+#pragma unroll
+        for (int u = 0; u < kLeafU; ++u) t[u] = x[r][u];
+#pragma unroll
+        for (int c = 0; c < W - kLeafU; ++c) x[r][c] = x[r][c + kLeafU];
+#pragma unroll
+        for (int u = 0; u < kLeafU; ++u) x[r][W - kLeafU + u] = t[u];
+    }
+}
+
+// One phase: columns [LW - W, LW - W + kLeafQ) in groups of kLeafU (a
+// runtime loop over fixed register indices, the window rotating by kLeafU
+// after each group); at its end the phase's columns sit, in order, at
+// [W - kLeafQ, W), outside the next window.
+template <typename T, int RPT, int NP, int W>
+__device__ __attribute__((always_inline)) inline bool leaf_phase(
+    const LeafArgs<T>& a, LeafShared<T>& sh, T (&x)[RPT][kLeafWidth<T>], int (&tag)[RPT],
+    int (&lab)[RPT], T (&lp)[RPT], int g)
+{
+    constexpr int LW = kLeafWidth<T>;
+    constexpr int QP = LW - W;
+    for (int base = QP; base < QP + kLeafQ; base += kLeafU) {
+        if (base < a.w && !leaf_step<T, RPT, NP, W, 0>(a, sh, x, tag, lab, lp, base, g))
+            return false;
+        if (base + 1 < a.w && !leaf_step<T, RPT, NP, W, 1>(a, sh, x, tag, lab, lp, base, g))
+            return false;
+        if (base + 2 < a.w && !leaf_step<T, RPT, NP, W, 2>(a, sh, x, tag, lab, lp, base, g))
+            return false;
+        if (base + 3 < a.w && !leaf_step<T, RPT, NP, W, 3>(a, sh, x, tag, lab, lp, base, g))
+            return false;
+        leaf_rotate<T, RPT, W>(x);
+    }
+    return true;
+}
+
+// Cooperative partial-pivot LU of an m x w leaf panel (w <= LW: 64 columns
+// in FP32, 32 in FP64).  Blocks b with b % stride == 0 hold rows: workgroup
+// g = b / stride loads rows [g * 512 * RPT, (g + 1) * 512 * RPT) of the
+// panel, RPT per thread, into registers for the whole leaf; the other
+// blocks exit at once.  The hardware deals consecutive blocks to the XCDs
+// in turn, so a stride of 8 keeps the exchanging workgroups on one XCD.
+// Rows never move between threads: each register row carries a label, its
+// current row index in the LAPACK sense, and a row exchange swaps two
+// labels.  At the end every row is stored at its label.  Per column j:
+//   1. local candidate: wave64 arg-max, then across the 8 waves in LDS
 //      (max |a(i, j)| over labels i >= j, smallest label on ties: IxAMAX);
-//   2. publish the record;
-//   3. gather every record of the column (one round trip once all are out);
-//   4. the global pivot p (every thread reduces the G records itself),
+//   2. publish the record (the candidate row in column order);
+//   3. apply column j - 1's update to the columns after j of own rows while
+//      the records travel;
+//   4. gather every record of the column (one round trip once all are out);
+//   5. the global pivot p (every thread reduces the G records itself),
 //      relabel j <-> p, scale by the reciprocal of the pivot (|pivot| >=
-//      sfmin, else divide, as LAPACK xGETF2) and apply the rank-1 update to
-//      own rows.
-// Workgroup 0 keeps each column's pivot row: they are the leaf's first rows,
-// used by the epilogue (inverse of the unit-lower block).
+//      sfmin, else divide, as LAPACK xGETF2) and update column j + 1 only.
+// Each element takes its updates in pivot order with the same fused
+// multiply-add, so the factors are those of the plain column-by-column
+// elimination.  The live register window shrinks by kLeafQ columns per phase
+// (finished columns leave it), which bounds the update work.  Workgroup 0
+// keeps each column's pivot row: they are the leaf's first rows, used by the
+// epilogue (inverse of the unit-lower block).
 template <typename T, int RPT, int NP>
 __global__ __launch_bounds__(kLeafThreads) void getf2_leaf_kernel(LeafArgs<T> a)
 {
+    constexpr int LW = kLeafWidth<T>;
     constexpr int ROWS = kLeafThreads * RPT;
     if (static_cast<int>(blockIdx.x) % a.stride != 0) return;
     const int tid = threadIdx.x;
@@ -471,39 +570,43 @@ __global__ __launch_bounds__(kLeafThreads) void getf2_leaf_kernel(LeafArgs<T> a)
 
     // Columns never reached (an expired wait) stay unexchanged: identity local
     // pivots, and ipiv entries that are valid row indices.
-    if (tid < kLeafW) sh.piv[tid] = tid;
+    if (tid < LW) sh.piv[tid] = tid;
     if (tid == 0) sh.abort = 0;
     if (g == 0 && tid < w) a.ipiv[a.c0 + tid] = a.c0 + tid + 1;
-    T x[RPT][kLeafW];
+    T x[RPT][LW];
     int tag[RPT];
     int lab[RPT];
+    T lp[RPT];
 #pragma unroll
     for (int r = 0; r < RPT; ++r) {
         const int lrow = base + r * kLeafThreads + tid;
         const bool in = lrow < m;
 #pragma unroll
-        for (int c = 0; c < kLeafW; ++c)
+        for (int c = 0; c < LW; ++c)
             x[r][c] = (in && c < w) ? a.A[off(a.c0 + lrow, a.c0 + c, a.lda)] : T(0);
         tag[r] = in ? a.orig[a.c0 + lrow] : -1;
         lab[r] = in ? lrow : kIdxNone;
+        lp[r] = T(0);
     }
 
-    
-    int j0 = 0;
-    for (; j0 < w; j0 += kLeafU) {
-        if (!leaf_column<T, RPT, NP, 0>(a, sh, x, tag, lab, j0, g)) return;
-        if (j0 + 1 < w && !leaf_column<T, RPT, NP, 1>(a, sh, x, tag, lab, j0, g)) return;
-        rotate_rows<T, RPT>(x);
-    }
-    for (; j0 < kLeafW; j0 += kLeafU) rotate_rows<T, RPT>(x);
+    bool ok = leaf_phase<T, RPT, NP, LW>(a, sh, x, tag, lab, lp, g);
+    if constexpr (LW > kLeafQ) ok = ok && leaf_phase<T, RPT, NP, LW - kLeafQ>(a, sh, x, tag, lab, lp, g);
+    if constexpr (LW > 2 * kLeafQ) ok = ok && leaf_phase<T, RPT, NP, LW - 2 * kLeafQ>(a, sh, x, tag, lab, lp, g);
+    if constexpr (LW > 3 * kLeafQ) ok = ok && leaf_phase<T, RPT, NP, LW - 3 * kLeafQ>(a, sh, x, tag, lab, lp, g);
+    static_assert(LW <= 4 * kLeafQ && LW % kLeafQ == 0, "leaf of one to four phases");
+    if (!ok) return;
 
+    // after the last phase register i holds column leaf_index<LW>(0, i)
 #pragma unroll
     for (int r = 0; r < RPT; ++r) {
         if (lab[r] != kIdxNone) {
             const int row = a.c0 + lab[r];
 #pragma unroll
-            for (int c = 0; c < kLeafW; ++c)
-                if (c < w) a.A[off(row, a.c0 + c, a.lda)] = x[r][c];
+            for (int i = 0; i < LW; ++i) {
+                constexpr int NPH = LW / kLeafQ;
+                const int c = kLeafQ * (NPH - 1 - i / kLeafQ) + i % kLeafQ;
+                if (c < w) a.A[off(row, a.c0 + c, a.lda)] = x[r][i];
+            }
             a.orig[row] = tag[r];
         }
     }
@@ -525,33 +628,32 @@ __global__ __launch_bounds__(kLeafThreads) void getf2_leaf_kernel(LeafArgs<T> a)
         a.mv_src[tid] = (pos != dst) ? a.c0 + pos : -1;
         a.mv_dst[tid] = a.c0 + dst;
     }
-    // (b) Inverse of the unit-lower w x w diagonal block (identity rows past w):
-    //     wave v < 4, half h solves L x = e_c for columns c = 8 v + 4 h + q by
-    //     forward substitution, lane (l % 32) holding row l % 32.
-    if (a.inv == nullptr || tid >= 256) return;
+    // (b) Inverse of the unit-lower w x w diagonal block (identity rows past
+    //     w) by forward substitution: wave v solves L x = e_c for its LW / 8
+    //     columns c, lane i < LW holding row i.
+    if (a.inv == nullptr) return;
+    constexpr int CPW = LW / kLeafWaves;
     const int lane = tid & 63;
-    const int i = lane & 31;
-    const int half = lane & 32;
-    T xv[4];
+    const int i = lane;
+    const int wv = tid >> 6;
+    T xv[CPW];
 #pragma unroll
-    for (int q = 0; q < 4; ++q) {
-        const int c = (tid >> 6) * 8 + (half >> 3) + q;
-        xv[q] = (i == c) ? T(1) : T(0);
-    }
+    for (int q = 0; q < CPW; ++q) xv[q] = (i == wv * CPW + q) ? T(1) : T(0);
+    for (int k = 0; k < LW; ++k) {
+        const T lik = (i < w && i < LW && k < i) ? sh.lb[i][k] : T(0);
 #pragma unroll
-    for (int k = 0; k < kLeafW; ++k) {
-        const T lik = (i < w && k < i) ? sh.lb[i][k] : T(0);
-#pragma unroll
-        for (int q = 0; q < 4; ++q) {
-            const T xk = __shfl(xv[q], half + k);
+        for (int q = 0; q < CPW; ++q) {
+            const T xk = __shfl(xv[q], k);
             if (i > k) xv[q] -= lik * xk;
         }
     }
+    if (i < LW) {
 #pragma unroll
-    for (int q = 0; q < 4; ++q) {
-        const int c = (tid >> 6) * 8 + (half >> 3) + q;
-        a.inv[i + 64 * c] = xv[q];
-        if (a.inv_zero != nullptr) a.inv_zero[i + 64 * c] = T(0);
+        for (int q = 0; q < CPW; ++q) {
+            const int c = wv * CPW + q;
+            a.inv[i + 64 * c] = xv[q];
+            if (a.inv_zero != nullptr) a.inv_zero[i + 64 * c] = T(0);
+        }
     }
 }
 
@@ -575,10 +677,13 @@ static void launch_leaf(mifact_handle h, LeafArgs<T>& args)
 template <typename T, int RPT>
 static void launch_leaf_np(mifact_handle h, LeafArgs<T>& args)
 {
-    // words gathered per thread: up to 16 records in the common case
+    // words gathered per thread: up to 16, 32 or kMaxG records
     constexpr int kSmall = (16 * Rec<T>::RW + kLeafThreads - 1) / kLeafThreads;
+    constexpr int kMid = (32 * Rec<T>::RW + kLeafThreads - 1) / kLeafThreads;
     constexpr int kLarge = (kMaxG * Rec<T>::RW + kLeafThreads - 1) / kLeafThreads;
+    static_assert(kLarge <= 32, "poll mask is 32 bits");
     if (args.G <= 16) launch_leaf<T, RPT, kSmall>(h, args);
+    else if (args.G <= 32) launch_leaf<T, RPT, kMid>(h, args);
     else launch_leaf<T, RPT, kLarge>(h, args);
 }
 
@@ -588,11 +693,12 @@ void getf2_leaf(mifact_handle h, T* A, long long lda, int n, int c0, int w, int*
 {
     const int m = n - c0;
     if (w <= 0 || m <= 0) return;
-    if (w > kLeafW) {
-        report_status(h, mifact_status_invalid_value, "getf2_leaf: w > 32");
+    if (w > kLeafWidth<T>) {
+        report_status(h, mifact_status_invalid_value, "getf2_leaf: w above the leaf width");
         return;
     }
-    static_assert(2 * Rec<double>::PARITY * 8 <= kExchangeBytes / 2,
+    static_assert(2 * Rec<float>::PARITY * 8 <= kExchangeBytes / 2
+                      && 2 * Rec<double>::PARITY * 8 <= kExchangeBytes / 2,
                   "leaf exchange records exceed their half of the exchange area");
     Workspace ws = workspace(h);
     LeafArgs<T> args;
@@ -614,13 +720,9 @@ void getf2_leaf(mifact_handle h, T* A, long long lda, int n, int c0, int w, int*
     args.inv_zero = inv_zero;
     args.mv_src = mv_src;
     args.mv_dst = mv_dst;
-    // Rows per thread: grow (FP32 up to 4, FP64 up to 2 x 32 values per
-    // thread in the registers of a 512-thread workgroup) while more than 16
-    // workgroups would have to exchange.
-    constexpr int max_rpt = sizeof(T) == 4 ? 4 : 2;
-    int rpt = 1;
-    while (rpt < max_rpt && ceil_div(m, kLeafThreads * rpt) > 16) rpt *= 2;
-    args.G = ceil_div(m, kLeafThreads * rpt);
+    // One row per thread (LW values in registers; a second row would not
+    // fit beside the rotating window without spills).
+    args.G = ceil_div(m, kLeafThreads);
     if (args.G > kMaxG) {
         report_status(h, mifact_status_invalid_value, "getf2_leaf: panel too tall");
         return;
@@ -630,9 +732,7 @@ void getf2_leaf(mifact_handle h, T* A, long long lda, int n, int c0, int w, int*
     int stride = 8;
     while (stride > 1 && args.G * stride > h->num_cu) stride /= 2;
     args.stride = stride;
-    if (rpt == 1) launch_leaf_np<T, 1>(h, args);
-    else if (rpt == 2) launch_leaf_np<T, 2>(h, args);
-    else if constexpr (max_rpt >= 4) launch_leaf_np<T, 4>(h, args);
+    launch_leaf_np<T, 1>(h, args);
 }
 
 // Completes the inverse of a 64 x 64 unit-lower diagonal block from the

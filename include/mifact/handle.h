@@ -72,8 +72,8 @@ mifact_status mifact_reserve_workspace(mifact_handle handle, int n);
 
 /* FP32 update-GEMM tile configuration used by every routine on this handle
  * (a tuning knob for the autotuner): 0 = automatic (default), 1 = 128x128
- * tiles with 16-deep K slices, 2 = 128x128 / 32-deep, 3 = 64x64 / 16-deep,
- * 4 = 256x256 / 16-deep. */
+ * tiles with 16-deep K slices (four waves), 2 = 128x128 / 32-deep, 3 = 64x64
+ * / 16-deep, 4 = 256x256 / 16-deep on eight waves. */
 mifact_status mifact_set_gemm_config(mifact_handle handle, int config);
 
 /* Library version string. */
