@@ -31,12 +31,17 @@ import time
 
 METRIC = "POTRF/GETRF TFLOP/s @n=32768; mixed+IR vs FP32 solve time, same accuracy"
 FP32_PEAK_TFLOPS = 157.3      # MI355X dense FP32 (MFMA)
-# HBM bytes per dispatch of the large-tile update GEMM, averaged over its
+# HBM bytes per dispatch of the dominant update-GEMM kernel, averaged over its
 # dispatches in one bench step, keyed by (n, potrf nb, potrf b, getrf nb,
-# getrf b): calibrated from rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE runs of
-# this script, scaled by the same counters' reading of a copy of known size.
-GEMM_TRAFFIC = {}
-GEMM_TRAFFIC_SOURCE = ""
+# getrf b, profiler stage): calibrated from rocprofv3 --pmc FETCH_SIZE /
+# WRITE_SIZE runs of this script, scaled by the same counters' reading of a
+# device copy of known size (FETCH_SIZE x 2.0, WRITE_SIZE x 1.0).
+GEMM_TRAFFIC = {
+    # 256 x 256 NN kernel, 7 dispatches per GETRF: 8.77 GB read + 0.62 GB written
+    (32768, 8192, 256, 16384, 4096, "gemm_update_256"): 9.39e9,
+}
+GEMM_TRAFFIC_SOURCE = ("calibrated, not live: rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE of "
+                       "this bench's GETRF step (profiles/getrf_n32768_pmc.md)")
 HBM_PEAK_GBS = 8000.0
 
 
@@ -477,11 +482,11 @@ def main():
             run(r)
             torch.cuda.synchronize()
             log(f"warmup {r} {i} done")
-    # Timed steps: the stage profiler brackets only the dominant kernel (the
-    # large-tile trailing-update GEMM, a few hundred launches per step), so
-    # its event pairs do not perturb the step time.
+    # Timed steps: the stage profiler brackets only the large-tile
+    # trailing-update GEMMs (a few hundred launches per step), so its event
+    # pairs do not perturb the step time.
     if not args.no_profile:
-        _lib.profile_begin(h, ["gemm_update"])
+        _lib.profile_begin(h, ["gemm_update", "gemm_update_256"])
     for i in range(args.steps):
         for r in routines:
             work.copy_(inputs[r])
@@ -538,22 +543,35 @@ def main():
     value = total_flops / (total_ms * 1e-3) / 1e12
 
     roofline = None
-    g = dominant.get("gemm_update")
-    if g and g["ms"] > 0:
+    # the dominant kernel: the large-tile update GEMM stage with the most time
+    kernel_names = {
+        "gemm_update_256": "mifact::gemm_kernel<float,256,256,16,4,2,NN,full,vec> "
+                           "(eight waves, MFMA 32x32x2 f32): GETRF trailing updates",
+        "gemm_update": "mifact::gemm_kernel<float,128,128,16,2,2,...> (four waves, MFMA "
+                       "32x32x2 f32): POTRF trapezoid and mid-size GETRF updates",
+    }
+    ranked = sorted((k for k in kernel_names if dominant.get(k) and dominant[k]["ms"] > 0),
+                    key=lambda k: -dominant[k]["ms"])
+    if ranked:
+        g = dominant[ranked[0]]
         achieved = g["flops"] / (g["ms"] * 1e-3) / 1e12
         roofline = {"bound": "mfma", "achieved": achieved, "peak": FP32_PEAK_TFLOPS,
                     "unit": "TFLOP/s", "frac": achieved / FP32_PEAK_TFLOPS, "traffic": None,
-                    "kernel": "mifact::gemm_kernel<float,128,128,16,...> trailing updates "
-                              "(MFMA 32x32x2 f32), all launches of the timed steps",
-                    "traffic_kernel": "gemm_kernel<float,128,128,16,NT,lower>: POTRF outer "
-                                      "trapezoid updates, HBM bytes per dispatch",
+                    "kernel": kernel_names[ranked[0]] + ", all launches of the timed steps",
+                    "stage": ranked[0],
                     "launches": g["calls"],
                     "avg_launch_ms": g["ms"] / max(g["calls"], 1),
                     "flops_per_launch": g["flops"] / max(g["calls"], 1)}
-    if roofline is not None:
-        # calibrated, not live: rocprofv3 --pmc runs of this bench command;
-        # null for a block configuration that was not measured
-        key = (n, pnb, pb, gnb, gb)
+        if len(ranked) > 1:
+            o = dominant[ranked[1]]
+            roofline["next_kernel"] = {
+                "kernel": kernel_names[ranked[1]], "launches": o["calls"],
+                "avg_launch_ms": o["ms"] / max(o["calls"], 1),
+                "achieved": o["flops"] / (o["ms"] * 1e-3) / 1e12}
+        # calibrated, not live: rocprofv3 --pmc runs of this bench command
+        # (HBM bytes per launch of the dominant kernel); null for a block
+        # configuration that was not measured
+        key = (n, pnb, pb, gnb, gb, ranked[0])
         roofline["traffic"] = GEMM_TRAFFIC.get(key)
         roofline["traffic_source"] = (GEMM_TRAFFIC_SOURCE if key in GEMM_TRAFFIC
                                       else "not measured for this block configuration")

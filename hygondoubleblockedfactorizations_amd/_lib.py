@@ -217,7 +217,7 @@ class StageStats(ctypes.Structure):
 
 
 STAGES = ["gemm", "potf2", "trsm_leaf", "getf2_leaf", "row_moves", "perm", "gemm_lowp",
-          "solve", "batched", "gemm_update"]
+          "solve", "batched", "gemm_update", "gemm_update_256"]
 
 
 def profile_begin(handle, stages=None):

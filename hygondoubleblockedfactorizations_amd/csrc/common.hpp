@@ -25,6 +25,7 @@ enum Stage {
     kStageSolve,
     kStageBatched,
     kStageGemmUpdate,
+    kStageGemmUpdate256,
     kStageCount
 };
 

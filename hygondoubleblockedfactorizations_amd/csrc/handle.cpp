@@ -163,7 +163,7 @@ mifact_status mifact_profile_end(mifact_handle h, mifact_stage_stats* stats, int
     h->profiling = false;
     static const char* names[mifact::kStageCount] = {"gemm", "potf2", "trsm_leaf", "getf2_leaf",
                                                      "row_moves", "perm", "gemm_lowp", "solve",
-                                                     "batched", "gemm_update"};
+                                                     "batched", "gemm_update", "gemm_update_256"};
     mifact_stage_stats acc[mifact::kStageCount];
     for (int s = 0; s < mifact::kStageCount; ++s) {
         std::snprintf(acc[s].name, sizeof(acc[s].name), "%s", names[s]);
