@@ -6,9 +6,11 @@
 // 32-column panels, repeat until the queue is drained.  Per panel:
 //   POTRF: POTF2 of the 32 x 32 diagonal block in LDS, row-parallel TRSM of
 //          the rows below, lower-trapezoid SYRK of the trailing matrix;
-//   GETRF: column-by-column partial pivoting on the panel (smallest index on
-//          ties, LAPACK sgetf2 semantics), full-row exchanges,
-//          column-parallel unit-lower TRSM of the block row, GEMM update.
+//   GETRF: the panel in LDS, column-by-column partial pivoting (smallest
+//          index on ties, LAPACK sgetf2 semantics), the panel's row
+//          interchanges applied to the other columns in order,
+//          column-parallel unit-lower TRSM of the block row, GEMM update
+//          (orders up to 1024).
 // The trailing updates run as 64 x 64 MFMA tiles (v_mfma_f32_32x32x2_f32,
 // or v_mfma_f32_32x32x16_bf16/_f16 with FP32 accumulation for the
 // mixed-precision variants).  Same arithmetic contract as the large drivers
@@ -208,28 +210,52 @@ __device__ void chol_matrix(float* A, long long lda, int n, int* info)
 
 // ---------------------------------------------------------------- LU
 
+constexpr int kLuMaxOrder = 1024;   // the whole panel (up to 1024 x 32) sits in LDS
+constexpr int kPanelLd = kNB + 1;   // LDS row stride: conflict-free column walks
+
+// Right-looking LU with partial pivoting of one matrix by one workgroup
+// (LAPACK sgetrf semantics: IxAMAX first of ties, reciprocal scaling when
+// |pivot| >= FLT_MIN, the row interchanges of each panel applied in order to
+// every other column, info = first zero pivot).  Per 32-column panel:
+//   1. the m x kb panel (m = n - k <= 1024 rows) is copied into LDS and
+//      factored there column by column (arg-max, row exchange within the
+//      panel, scaling, rank-1 update of the panel's remaining columns);
+//   2. the factored panel is written back; the panel's interchanges are
+//      applied to the columns left and right of it (one thread per column,
+//      in order);
+//   3. U12 := L11^{-1} A12 (unit lower, L11 from LDS; one thread per column);
+//   4. A22 -= L21 U12 on MFMA tiles.
+// Every global access is to (row, column) of the n x n matrix with both
+// below n (the caller checks lda >= n), every LDS access below its bounds.
 template <int PREC>
 __device__ void lu_matrix(float* A, long long lda, int n, int* ipiv, int* info)
 {
+    __shared__ float pnl[kLuMaxOrder][kPanelLd];
     __shared__ float wv[4];
     __shared__ int wi[4];
-    __shared__ float prow[kNB];
-    __shared__ float l11[kNB][kNB + 1];
+    __shared__ int piv_s[kNB];
     __shared__ TileShared ts;
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     constexpr float sfmin = FLT_MIN;
     for (int k = 0; k < n; k += kNB) {
         const int kb = imin(kNB, n - k);
-        // panel columns k .. k + kb
-        for (int j = k; j < k + kb; ++j) {
-            // arg max |A(i, j)|, i >= j (smallest index on ties; NaN never wins)
+        const int m = n - k;
+        // 1. panel -> LDS (coalesced down each column)
+        for (int e = tid; e < m * kNB; e += 256) {
+            const int r = e % m, c = e / m;
+            pnl[r][c] = (c < kb) ? A[off(k + r, k + c, lda)] : 0.0f;
+        }
+        __syncthreads();
+        for (int j = 0; j < kb; ++j) {
+            // arg max |pnl(r, j)| over r >= j: first index among equal
+            // magnitudes, NaN never wins
             float bv = -1.0f;
-            int bi = n;
-            for (int i = j + tid; i < n; i += 256) {
-                const float v = fabsf(A[off(i, j, lda)]);
+            int bi = m;
+            for (int r = j + tid; r < m; r += 256) {
+                const float v = fabsf(pnl[r][j]);
                 if (v > bv) {
                     bv = v;
-                    bi = i;
+                    bi = r;
                 }
             }
 #pragma unroll
@@ -248,52 +274,59 @@ __device__ void lu_matrix(float* A, long long lda, int n, int* ipiv, int* info)
             __syncthreads();
             float pv = wv[0];
             int p = wi[0];
+#pragma unroll
             for (int w = 1; w < 4; ++w)
                 if (wv[w] > pv || (wv[w] == pv && wi[w] < p)) {
                     pv = wv[w];
                     p = wi[w];
                 }
-            if (p >= n) p = j;   // all NaN: keep the diagonal
-            if (tid == 0) ipiv[j] = p + 1;
-            const float piv = A[off(p, j, lda)];
-            const bool zero = (piv == 0.0f);
-            if (zero && tid == 0 && *info == 0) *info = j + 1;
-            __syncthreads();     // every thread has read the pivot before rows move
-            // exchange rows j and p over all n columns
-            if (p != j) {
-                for (int c = tid; c < n; c += 256) {
-                    const float t = A[off(j, c, lda)];
-                    A[off(j, c, lda)] = A[off(p, c, lda)];
-                    A[off(p, c, lda)] = t;
-                }
+            if (p < j || p >= m) p = j;   // no candidate (all NaN): keep the diagonal
+            const float piv = pnl[p][j];
+            if (tid == 0) {
+                ipiv[k + j] = k + p + 1;
+                piv_s[j] = p;
+                if (piv == 0.0f && *info == 0) *info = k + j + 1;
+            }
+            __syncthreads();   // every thread has read the pivot before the rows move
+            if (p != j && tid < kNB) {
+                const float t = pnl[j][tid];
+                pnl[j][tid] = pnl[p][tid];
+                pnl[p][tid] = t;
             }
             __syncthreads();
-            // pivot row within the panel (columns j+1 .. k+kb)
-            if (tid < kNB) prow[tid] = (j + 1 + tid < k + kb) ? A[off(j, j + 1 + tid, lda)] : 0.0f;
-            __syncthreads();
-            // scale the column and update the rest of the panel
-            if (!zero) {
+            if (piv != 0.0f) {
                 const bool recip = fabsf(piv) >= sfmin;
                 const float rinv = 1.0f / piv;
-                const int ncol = k + kb - j - 1;
-                for (int i = j + 1 + tid; i < n; i += 256) {
-                    float l = A[off(i, j, lda)];
-                    l = recip ? l * rinv : l / piv;
-                    A[off(i, j, lda)] = l;
-                    for (int c = 0; c < ncol; ++c) A[off(i, j + 1 + c, lda)] -= l * prow[c];
+                for (int r = j + 1 + tid; r < m; r += 256) {
+                    const float x = pnl[r][j];
+                    const float l = recip ? x * rinv : x / piv;
+                    pnl[r][j] = l;
+                    for (int c = j + 1; c < kb; ++c) pnl[r][c] -= l * pnl[j][c];
                 }
             }
             __syncthreads();
         }
-        const int c1 = k + kb;
-        const int m = n - c1;
-        if (m <= 0) continue;
-        // U12 := L11^{-1} A12 (unit lower), one thread per column
-        for (int e = tid; e < kNB * kNB; e += 256) {
-            const int i = e % kNB, j = e / kNB;
-            l11[i][j] = (i < kb && j < kb) ? A[off(k + i, k + j, lda)] : 0.0f;
+        // 2. panel -> global; the panel's interchanges to the other columns
+        for (int e = tid; e < m * kb; e += 256) {
+            const int r = e % m, c = e / m;
+            A[off(k + r, k + c, lda)] = pnl[r][c];
+        }
+        for (int t = tid; t < n - kb; t += 256) {
+            const int c = t < k ? t : t + kb;
+            for (int jj = 0; jj < kb; ++jj) {
+                const int p = piv_s[jj];
+                if (p != jj) {
+                    const float x = A[off(k + jj, c, lda)];
+                    A[off(k + jj, c, lda)] = A[off(k + p, c, lda)];
+                    A[off(k + p, c, lda)] = x;
+                }
+            }
         }
         __syncthreads();
+        const int c1 = k + kb;
+        const int mr = n - c1;
+        if (mr <= 0) continue;
+        // 3. U12 := L11^{-1} A12 (unit lower L11 = pnl rows 0..kb)
         for (int c = c1 + tid; c < n; c += 256) {
             float x[kNB];
 #pragma unroll
@@ -302,7 +335,7 @@ __device__ void lu_matrix(float* A, long long lda, int n, int* ipiv, int* info)
             for (int i = 1; i < kNB; ++i) {
                 float v = x[i];
 #pragma unroll
-                for (int q = 0; q < i; ++q) v -= l11[i][q] * x[q];
+                for (int q = 0; q < i; ++q) v -= pnl[i][q] * x[q];
                 x[i] = v;
             }
 #pragma unroll
@@ -310,16 +343,16 @@ __device__ void lu_matrix(float* A, long long lda, int n, int* ipiv, int* info)
                 if (i < kb) A[off(k + i, c, lda)] = x[i];
         }
         __syncthreads();
-        // A22 -= L21 U12
+        // 4. A22 -= L21 U12
         float* C = A + off(c1, c1, lda);
         const float* L21 = A + off(c1, k, lda);
         const float* U12 = A + off(k, c1, lda);
         auto la = [&](int i, int kk) { return L21[off(i, kk, lda)]; };
-        auto rb = [&](int kk, int j) { return U12[off(kk, j, lda)]; };
-        const int tiles = (m + kT - 1) / kT;
+        auto rb = [&](int kk, int jc) { return U12[off(kk, jc, lda)]; };
+        const int tiles = (mr + kT - 1) / kT;
         for (int tj = 0; tj < tiles; ++tj)
             for (int ti = 0; ti < tiles; ++ti)
-                tile_update<PREC, false>(C, lda, m, m, ti * kT, tj * kT, kb, la, rb, ts);
+                tile_update<PREC, false>(C, lda, mr, mr, ti * kT, tj * kT, kb, la, rb, ts);
     }
 }
 
@@ -342,13 +375,18 @@ __global__ __launch_bounds__(256) void batched_kernel(BatchArgs a)
         if (q >= a.batch) return;            // queue drained: every workgroup exits
         const int id = a.order[q];
         const int n = a.n[id];
-        if (n <= 0 || n > kMaxOrder) continue;
-        float* A = a.A + a.offsets[id];
         const long long lda = a.lda[id];
         int* info = a.info + id;
+        // orders beyond the path's limit or lda < n: LAPACK's illegal-argument
+        // info, nothing touched
+        if (n <= 0 || n > (LU ? kLuMaxOrder : kMaxOrder) || lda < n) {
+            if (threadIdx.x == 0) *info = n == 0 ? 0 : -1;
+            continue;
+        }
+        float* A = a.A + a.offsets[id];
         if (threadIdx.x == 0) *info = 0;
         __syncthreads();
-        if (LU) lu_matrix<PREC>(A, lda, n, a.ipiv + a.ipiv_offsets[id], info);
+        if constexpr (LU) lu_matrix<PREC>(A, lda, n, a.ipiv + a.ipiv_offsets[id], info);
         else chol_matrix<PREC>(A, lda, n, info);
     }
 }
@@ -620,7 +658,8 @@ void launch_batched(mifact_handle h, bool lu, LowPrecision prec, int batch, floa
                        s.counter);
     MIFACT_LAUNCHED(h);
     BatchArgs a{A, offsets, n, lda, ipiv, ipiv_offsets, info, s.order, s.counter, batch};
-    const int grid = imax(1, imin(batch, h->num_cu * 4));
+    // the LU workgroup holds a 1024 x 32 panel in LDS: one per CU
+    const int grid = imax(1, imin(batch, h->num_cu * (lu ? 1 : 4)));
     StageScope scope(h, kStageBatched, 0.0, 0.0);
     auto go = [&](auto kern) {
         hipLaunchKernelGGL(kern, dim3(grid), dim3(256), 0, h->stream, a);
@@ -639,12 +678,6 @@ void launch_batched(mifact_handle h, bool lu, LowPrecision prec, int batch, floa
 
 bool valid_prec(int p) { return p == kPrecFP32 || p == kPrecBF16 || p == kPrecFP16; }
 
-// The batched LU kernel is switched off: its first GPU run ended in a memory
-// aperture fault whose cause has not been identified (every index of
-// lu_matrix checks out against the matrix bounds on reading).  Until it is,
-// the LU entry points report mifact_status_invalid_value without launching.
-constexpr bool kBatchedLuEnabled = false;
-
 void batched_entry(mifact_handle h, bool lu, int prec, int batch, float* A,
                    const long long* offsets, const int* n, const int* lda, int* ipiv,
                    const long long* ipiv_offsets, int* info)
@@ -654,10 +687,6 @@ void batched_entry(mifact_handle h, bool lu, int prec, int batch, float* A,
         return;
     }
     if (batch <= 0) return;
-    if (lu && !kBatchedLuEnabled) {
-        report_status(h, mifact_status_invalid_value, "batched LU is not available in this build");
-        return;
-    }
     if (!valid_prec(prec) || A == nullptr || offsets == nullptr || n == nullptr
         || lda == nullptr || info == nullptr || (lu && (ipiv == nullptr || ipiv_offsets == nullptr))) {
         report_status(h, mifact_status_invalid_value, "batched factorization: arguments");
@@ -719,10 +748,6 @@ mifact_status mifact_gesv_batched_refine(mifact_handle handle, int sym,
     }
     if (batch <= 0) return mifact_status_success;
     const bool lu = sym == 0;
-    if (lu && !kBatchedLuEnabled) {
-        report_status(h, mifact_status_invalid_value, "batched LU is not available in this build");
-        return mifact_status_invalid_value;
-    }
     if (!valid_prec(precision) || iterations < 0 || vec_elems <= 0 || A == nullptr
         || F == nullptr || offsets == nullptr || n == nullptr || lda == nullptr || B == nullptr
         || X == nullptr || vec_offsets == nullptr || berr == nullptr || info == nullptr

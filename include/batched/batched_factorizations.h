@@ -17,7 +17,10 @@
  * device arrays of `batch` entries.  info[i] (device) receives 0 or the
  * LAPACK info of matrix i.  ipiv for matrix i is at ipiv + ipiv_offsets[i]
  * (1-based row indices local to the matrix, LAPACK convention).
- * Orders up to 4096 are supported; every launch is on the handle's stream.
+ * Orders up to 4096 are supported for Cholesky and up to 1024 for LU (the
+ * LU panel is held in LDS); a matrix beyond the limit, or with lda < n,
+ * gets info = -1 and is left untouched.  Every launch is on the handle's
+ * stream.
  */
 #pragma once
 

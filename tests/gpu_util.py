@@ -157,3 +157,41 @@ def getrf_residual_gpu(A_buf, LU_buf, ipiv, bs=8192):
         num += float((acc ** 2).sum())
         del acc
     return float(np.sqrt(num / den))
+
+
+def round_to(x, prec):
+    """x (FP32 array) rounded to BF16 or FP16 and back (nearest even)."""
+    import torch
+    t = torch.from_numpy(np.ascontiguousarray(x, np.float32))
+    dt = torch.bfloat16 if prec == "bf16" else torch.float16
+    return t.to(dt).to(torch.float32).numpy()
+
+
+def emulated_lowp_lu(A, prec, nb=32):
+    """CPU emulation of the batched kernel's mixed-precision LU: 32-column
+    panels factored in FP32 with partial pivoting (LAPACK order), the
+    panel's interchanges applied to the other columns, U12 = L11^{-1} A12 in
+    FP32, and A22 -= L21 U12 with both operands rounded to `prec` and the
+    products accumulated in FP32.  Returns (LU, ipiv 1-based)."""
+    a = np.array(A, np.float32, order="F")
+    n = a.shape[1]
+    ipiv = np.zeros(n, np.int32)
+    for k in range(0, n, nb):
+        kb = min(nb, n - k)
+        for j in range(k, k + kb):
+            p = j + int(np.argmax(np.abs(a[j:, j])))
+            ipiv[j] = p + 1
+            if p != j:
+                a[[j, p], :] = a[[p, j], :]
+            piv = a[j, j]
+            if piv != 0:
+                a[j + 1:, j] = a[j + 1:, j] * np.float32(1.0 / piv) if abs(piv) >= np.finfo(np.float32).tiny else a[j + 1:, j] / piv
+                a[j + 1:, j + 1:k + kb] -= np.outer(a[j + 1:, j], a[j, j + 1:k + kb]).astype(np.float32)
+        c1 = k + kb
+        if c1 < n:
+            L11 = np.tril(a[k:c1, k:c1], -1).astype(np.float64) + np.eye(kb)
+            a[k:c1, c1:] = np.linalg.solve(L11, a[k:c1, c1:].astype(np.float64)).astype(np.float32)
+            L21 = round_to(a[c1:, k:c1], prec).astype(np.float64)
+            U12 = round_to(a[k:c1, c1:], prec).astype(np.float64)
+            a[c1:, c1:] = (a[c1:, c1:].astype(np.float64) - L21 @ U12).astype(np.float32)
+    return a, ipiv

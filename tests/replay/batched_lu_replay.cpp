@@ -1,8 +1,11 @@
 // Host replay of the batched LU kernel's address arithmetic
 // (hygondoubleblockedfactorizations_amd/csrc/batched.hip: order_kernel,
-// batched_kernel<kPrecFP32, true>, lu_matrix, tile_update<kPrecFP32, false>).
+// batched_kernel<kPrecFP32, true>, lu_matrix with its panel in LDS,
+// tile_update<kPrecFP32, false>).
 //
-// Every phase between two workgroup barriers runs for tid = 0..255 in turn;
+// Every phase between two workgroup barriers runs for tid = 0..255 in turn
+// (or, where the phase's accesses do not depend on the thread split, as one
+// loop over the same index range);
 // the wave butterflies run lane by lane.  Every access to the packed matrix
 // buffer, to ipiv and to the workgroup arrays goes through a checked
 // accessor that counts accesses outside (a) the whole buffer, (b) the
@@ -24,7 +27,8 @@
 
 namespace {
 
-constexpr int kNB = 32, kT = 64, kMaxOrder = 4096, kBuckets = 256, kThreads = 256;
+constexpr int kNB = 32, kT = 64, kMaxOrder = 4096, kLuMaxOrder = 1024, kBuckets = 256,
+              kThreads = 256;
 
 struct Checked {
     std::vector<float> flat;
@@ -111,42 +115,49 @@ void tile_update(Checked& M, int c1, int k, int m, int r0, int c0, int kb)
 
 void lu_matrix(Checked& M, int n, int* info)
 {
-    float wv[4];
-    int wi[4];
-    Lds<float, 1, kNB> prow;
-    Lds<float, kNB, kNB + 1> l11;
-    prow.violations = l11.violations = &M.violations;
+    static Lds<float, kLuMaxOrder, kNB + 1> pnl;
+    Lds<int, 1, kNB> piv_s;
+    pnl.violations = piv_s.violations = &M.violations;
     for (int k = 0; k < n; k += kNB) {
         const int kb = std::min(kNB, n - k);
-        for (int j = k; j < k + kb; ++j) {
+        const int m = n - k;
+        // 1. panel -> LDS
+        for (int tid = 0; tid < kThreads; ++tid)
+            for (int e = tid; e < m * kNB; e += kThreads) {
+                const int r = e % m, c = e / m;
+                pnl.at(r, c) = (c < kb) ? M.A(k + r, k + c) : 0.0f;
+            }
+        for (int j = 0; j < kb; ++j) {
             float bv[kThreads];
             int bi[kThreads];
             for (int tid = 0; tid < kThreads; ++tid) {
                 bv[tid] = -1.0f;
-                bi[tid] = n;
-                for (int i = j + tid; i < n; i += kThreads) {
-                    const float v = std::fabs(M.A(i, j));
+                bi[tid] = m;
+                for (int r = j + tid; r < m; r += kThreads) {
+                    const float v = std::fabs(pnl.at(r, j));
                     if (v > bv[tid]) {
                         bv[tid] = v;
-                        bi[tid] = i;
+                        bi[tid] = r;
                     }
                 }
             }
+            float wv[4];
+            int wi[4];
             for (int w = 0; w < 4; ++w) {
                 for (int o = 32; o > 0; o >>= 1) {
                     float nv[64];
                     int ni[64];
                     for (int l = 0; l < 64; ++l) {
                         float v = bv[w * 64 + l];
-                        int i = bi[w * 64 + l];
+                        int ii = bi[w * 64 + l];
                         const float ov = bv[w * 64 + (l ^ o)];
                         const int oi = bi[w * 64 + (l ^ o)];
-                        if (ov > v || (ov == v && oi < i)) {
+                        if (ov > v || (ov == v && oi < ii)) {
                             v = ov;
-                            i = oi;
+                            ii = oi;
                         }
                         nv[l] = v;
-                        ni[l] = i;
+                        ni[l] = ii;
                     }
                     for (int l = 0; l < 64; ++l) {
                         bv[w * 64 + l] = nv[l];
@@ -163,46 +174,53 @@ void lu_matrix(Checked& M, int n, int* info)
                     pv = wv[w];
                     p = wi[w];
                 }
-            if (p >= n) p = j;
-            M.piv(j) = p + 1;
-            const float piv = M.A(p, j);
-            const bool zero = piv == 0.0f;
-            if (zero && *info == 0) *info = j + 1;
+            if (p < j || p >= m) p = j;
+            const float piv = pnl.at(p, j);
+            M.piv(k + j) = k + p + 1;
+            piv_s.at(0, j) = p;
+            if (piv == 0.0f && *info == 0) *info = k + j + 1;
             if (p != j)
-                for (int c = 0; c < n; ++c) std::swap(M.A(j, c), M.A(p, c));
-            for (int t = 0; t < kNB; ++t)
-                prow.at(0, t) = (j + 1 + t < k + kb) ? M.A(j, j + 1 + t) : 0.0f;
-            if (!zero) {
+                for (int t = 0; t < kNB; ++t) std::swap(pnl.at(j, t), pnl.at(p, t));
+            if (piv != 0.0f) {
                 const bool recip = std::fabs(piv) >= FLT_MIN;
                 const float rinv = 1.0f / piv;
-                const int ncol = k + kb - j - 1;
-                for (int i = j + 1; i < n; ++i) {
-                    float l = M.A(i, j);
-                    l = recip ? l * rinv : l / piv;
-                    M.A(i, j) = l;
-                    for (int c = 0; c < ncol; ++c) M.A(i, j + 1 + c) -= l * prow.at(0, c);
+                for (int r = j + 1; r < m; ++r) {
+                    const float x = pnl.at(r, j);
+                    const float l = recip ? x * rinv : x / piv;
+                    pnl.at(r, j) = l;
+                    for (int c = j + 1; c < kb; ++c) pnl.at(r, c) -= l * pnl.at(j, c);
                 }
             }
         }
-        const int c1 = k + kb, m = n - c1;
-        if (m <= 0) continue;
-        for (int e = 0; e < kNB * kNB; ++e) {
-            const int i = e % kNB, jj = e / kNB;
-            l11.at(i, jj) = (i < kb && jj < kb) ? M.A(k + i, k + jj) : 0.0f;
+        // 2. panel -> global; interchanges to the other columns
+        for (int e = 0; e < m * kb; ++e) {
+            const int r = e % m, c = e / m;
+            M.A(k + r, k + c) = pnl.at(r, c);
         }
+        for (int t = 0; t < n - kb; ++t) {
+            const int c = t < k ? t : t + kb;
+            for (int jj = 0; jj < kb; ++jj) {
+                const int pp = piv_s.at(0, jj);
+                if (pp != jj) std::swap(M.A(k + jj, c), M.A(k + pp, c));
+            }
+        }
+        const int c1 = k + kb, mr = n - c1;
+        if (mr <= 0) continue;
+        // 3. U12 := L11^{-1} A12
         for (int c = c1; c < n; ++c) {
             float x[kNB];
-            for (int i = 0; i < kNB; ++i) x[i] = i < kb ? M.A(k + i, c) : 0.0f;
-            for (int i = 1; i < kNB; ++i) {
-                float v = x[i];
-                for (int q = 0; q < i; ++q) v -= l11.at(i, q) * x[q];
-                x[i] = v;
+            for (int ii = 0; ii < kNB; ++ii) x[ii] = ii < kb ? M.A(k + ii, c) : 0.0f;
+            for (int ii = 1; ii < kNB; ++ii) {
+                float v = x[ii];
+                for (int q = 0; q < ii; ++q) v -= pnl.at(ii, q) * x[q];
+                x[ii] = v;
             }
-            for (int i = 0; i < kb; ++i) M.A(k + i, c) = x[i];
+            for (int ii = 0; ii < kb; ++ii) M.A(k + ii, c) = x[ii];
         }
-        const int tiles = (m + kT - 1) / kT;
+        // 4. A22 -= L21 U12
+        const int tiles = (mr + kT - 1) / kT;
         for (int tj = 0; tj < tiles; ++tj)
-            for (int ti = 0; ti < tiles; ++ti) tile_update(M, c1, k, m, ti * kT, tj * kT, kb);
+            for (int ti = 0; ti < tiles; ++ti) tile_update(M, c1, k, mr, ti * kT, tj * kT, kb);
     }
 }
 
@@ -261,7 +279,10 @@ int main(int argc, char** argv)
             continue;
         }
         const int nn = n[id];
-        if (nn <= 0 || nn > kMaxOrder) continue;
+        if (nn <= 0 || nn > kLuMaxOrder || lda[id] < nn) {
+            info[id] = nn == 0 ? 0 : -1;
+            continue;
+        }
         M.base = off[id];
         M.lda = lda[id];
         M.n = nn;

@@ -1,10 +1,9 @@
 """CPU replay of the batched LU kernel's address arithmetic on the exact
 inputs of the GPU batched-LU tests (tests/replay/batched_lu_replay.cpp).
 
-The batched LU kernel is switched off in the library after a memory
-aperture fault on its first GPU run.  This replay runs the kernel's phases
-thread by thread with bounds-checked accessors on the same packed batches
-(sizes, padded leading dimensions, offsets, ipiv offsets) and checks:
+The replay runs the kernel's phases thread by thread with bounds-checked
+accessors on the same packed batches (sizes, padded leading dimensions,
+offsets, ipiv offsets) and checks:
   * no access leaves the buffer, the matrix's lda x n block, its n x n part,
     its ipiv segment or the workgroup arrays;
   * the replay computes what the kernel must compute: ipiv identical to
@@ -54,7 +53,7 @@ def packed_case(lo, hi, batch):
     return mats, n, lda, off, piv_off, flat
 
 
-@pytest.mark.parametrize("lo,hi,batch", [(1, 96, 40), (64, 512, 16), (700, 1100, 3)])
+@pytest.mark.parametrize("lo,hi,batch", [(1, 96, 40), (64, 512, 16), (700, 1024, 3)])
 def test_batched_lu_addresses_in_range(replay, tmp_path, lo, hi, batch):
     mats, n, lda, off, piv_off, flat = packed_case(lo, hi, batch)
     inp, out = tmp_path / "in.bin", tmp_path / "out.bin"
@@ -81,3 +80,27 @@ def test_batched_lu_addresses_in_range(replay, tmp_path, lo, hi, batch):
         assert inf == 0 and info[i] == 0
         assert np.array_equal(p, pref), i
         assert O.getrf_residual(Fi, np.asfortranarray(A), p) <= 5e-4
+
+
+def test_batched_lu_order_limit(replay, tmp_path):
+    """Orders above the LU path's 1024 limit report info = -1 and leave the
+    matrix untouched; the rest of the batch factors normally."""
+    mats, n, lda, off, piv_off, flat = packed_case(1020, 1030, 3)
+    inp, out = tmp_path / "in.bin", tmp_path / "out.bin"
+    with open(inp, "wb") as f:
+        f.write(struct.pack("<iq", 3, flat.size))
+        for a in (n, lda, off, piv_off, flat):
+            f.write(a.tobytes())
+    r = subprocess.run([replay, str(inp), str(out)], capture_output=True, text=True)
+    assert r.returncode == 0, r.stdout + r.stderr
+    raw = open(out, "rb").read()
+    F = np.frombuffer(raw, np.float32, flat.size, 0)
+    ptot = int(np.maximum(n, 1).sum())
+    info = np.frombuffer(raw, np.int32, 3, 4 * (flat.size + ptot))
+    for i in range(3):
+        o, k, ld = int(off[i]), int(n[i]), int(lda[i])
+        if k > 1024:
+            assert info[i] == -1
+            assert np.array_equal(F[o:o + ld * k], flat[o:o + ld * k])
+        else:
+            assert info[i] == 0

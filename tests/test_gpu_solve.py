@@ -75,6 +75,39 @@ def round_lowp(x, prec, kaxis=1):
 PREC = {"fp32": 0, "bf16": 1, "fp16": 2, "fp8": 3}
 
 
+def emulated_mixed_lu(A, prec, nb=128):
+    """CPU model of the mixed-precision LU's rounding: nb-column panels
+    factored in FP32 with partial pivoting (LAPACK order), the panel's
+    interchanges applied to every other column, U12 = L11^{-1} A12, and the
+    Schur update A22 -= L21 U12 from operands rounded as the update GEMM packs
+    them (round_lowp), products accumulated exactly and stored to FP32.
+    Returns (LU, ipiv 1-based)."""
+    a = np.array(A, np.float32, order="F")
+    n = a.shape[1]
+    tiny = np.finfo(np.float32).tiny
+    ipiv = np.zeros(n, np.int32)
+    for k in range(0, n, nb):
+        kb = min(nb, n - k)
+        for j in range(k, k + kb):
+            p = j + int(np.argmax(np.abs(a[j:, j])))
+            ipiv[j] = p + 1
+            if p != j:
+                a[[j, p], :] = a[[p, j], :]
+            piv = a[j, j]
+            if piv != 0:
+                col = a[j + 1:, j]
+                a[j + 1:, j] = col * np.float32(1.0 / piv) if abs(piv) >= tiny else col / piv
+                a[j + 1:, j + 1:k + kb] -= np.outer(a[j + 1:, j], a[j, j + 1:k + kb]).astype(np.float32)
+        c1 = k + kb
+        if c1 < n:
+            L11 = np.tril(a[k:c1, k:c1], -1).astype(np.float64) + np.eye(kb)
+            a[k:c1, c1:] = np.linalg.solve(L11, a[k:c1, c1:].astype(np.float64)).astype(np.float32)
+            L21 = np.asarray(round_lowp(a[c1:, k:c1], prec, kaxis=1), np.float64)
+            U12 = np.asarray(round_lowp(a[k:c1, c1:], prec, kaxis=0), np.float64)
+            a[c1:, c1:] = (a[c1:, c1:].astype(np.float64) - L21 @ U12).astype(np.float32)
+    return a, ipiv
+
+
 def tile_pow2(rows, cols, seed):
     """Powers of two constant on 32 x 32 tiles (2^-2 .. 2^2): with them the
     FP8 tiles carry different scales (products stay multiples of 2^-4 below
@@ -419,9 +452,11 @@ def test_mixed_factorizations_residual(H, prec):
     of at most about 2u sum_k |l_ik||u_kj|; the rounding errors are independent
     and add like a random walk, so the normwise residual stays below
     u || |L||U| ||_F / ||A||_F, computed here from the returned factors (the
-    growth-aware form of the LU backward-error bound |dA| <= g_n |L||U|).  The
-    bound is checked for both drivers; a dropped or transposed update term
-    gives an O(1) residual, far above it."""
+    growth-aware form of the LU backward-error bound |dA| <= g_n |L||U|).  That
+    bound is tight for Cholesky (|L||L^T| ~ |A|) and checked there; for LU
+    |L||U| grows with n, so the GPU residual is held to 3x the residual of a
+    CPU model of the same rounding (emulated_mixed_lu).  A dropped or
+    transposed update term gives an O(1) residual, far above either."""
     n = 2048
     u = {"bf16": 2.0 ** -8, "fp16": 2.0 ** -11, "fp8": 2.0 ** -4}[prec]
     L = lib()
@@ -436,12 +471,21 @@ def test_mixed_factorizations_residual(H, prec):
     log("potrf mixed", prec, res, "bound", bound)
     assert res <= bound
     A = O.uniform_matrix(n, 62)
+    if prec == "fp8":
+        # e4m3 (u = 2^-4) leaves an O(1) residual on a plain uniform matrix at
+        # this order (the CPU model gives 2.7); a sqrt(n) diagonal shift
+        # (kappa ~ 10) keeps the comparison meaningful
+        A[np.arange(n), np.arange(n)] += np.float32(np.sqrt(n))
     dA, ipiv = dev(A), dev_int(n)
     L.double_blocked_getrf_mixed(H.ptr, PREC[prec], P(dA), n, n, 512, 128, 0, 0, P(ipiv), P(info))
     sync()
     assert int(info.cpu()[0]) == 0
     F = host(dA)
     res = O.getrf_residual(F, A, ipiv.cpu().numpy())
-    bound = u * _growth(F, True) / np.linalg.norm(A.astype(np.float64))
-    log("getrf mixed", prec, res, "bound", bound)
-    assert res <= bound
+    # LU: |L||U| grows with n for general matrices, so the growth-aware bound
+    # says little; the residual is held to 3x that of the CPU model of the
+    # same rounding (emulated_mixed_lu) instead
+    E, pe = emulated_mixed_lu(A, prec)
+    emu = O.getrf_residual(np.asfortranarray(E), A, pe)
+    log("getrf mixed", prec, res, "emulated", emu)
+    assert res <= 3 * emu
