@@ -130,6 +130,7 @@ template <int PREC>
 __device__ void chol_matrix(float* A, long long lda, int n, int* info)
 {
     __shared__ float d[kNB][kNB + 1];
+    __shared__ float xs[256][kNB + 1];
     __shared__ TileShared ts;
     __shared__ int failed;
     const int tid = threadIdx.x;
@@ -177,23 +178,21 @@ __device__ void chol_matrix(float* A, long long lda, int n, int* info)
         const int r1 = k + kb;
         const int m = n - r1;
         if (m <= 0) break;
-        // TRSM: A(r, k:k+kb) := A(r, k:k+kb) L^{-T}, one thread per row
-        for (int r = r1 + tid; r < n; r += 256) {
-            float x[kNB];
-#pragma unroll
-            for (int j = 0; j < kNB; ++j) x[j] = j < kb ? A[off(r, k + j, lda)] : 0.0f;
-#pragma unroll
-            for (int j = 0; j < kNB; ++j) {
-                if (j < kb) {
+        // TRSM: A(r, k:k+kb) := A(r, k:k+kb) L^{-T}, one thread per row, the
+        // row in the thread's own LDS slot (runtime loops: a register-resident
+        // row with unrolled loops spills)
+        for (int r0 = r1; r0 < n; r0 += 256) {
+            const int r = r0 + tid;
+            float* x = xs[tid];
+            if (r < n) {
+                for (int j = 0; j < kb; ++j) x[j] = A[off(r, k + j, lda)];
+                for (int j = 0; j < kb; ++j) {
                     float v = x[j];
-#pragma unroll
                     for (int c = 0; c < j; ++c) v -= x[c] * d[j][c];
                     x[j] = v / d[j][j];
                 }
+                for (int j = 0; j < kb; ++j) A[off(r, k + j, lda)] = x[j];
             }
-#pragma unroll
-            for (int j = 0; j < kNB; ++j)
-                if (j < kb) A[off(r, k + j, lda)] = x[j];
         }
         __syncthreads();
         // SYRK: A22 (lower) -= P P^T with P = A(r1:n, k:k+kb)
@@ -211,6 +210,7 @@ __device__ void chol_matrix(float* A, long long lda, int n, int* info)
 // ---------------------------------------------------------------- LU
 
 constexpr int kLuMaxOrder = 1024;   // the whole panel (up to 1024 x 32) sits in LDS
+static_assert(kLuMaxOrder >= kNB + 256, "U12 solve scratch rows inside the panel array");
 constexpr int kPanelLd = kNB + 1;   // LDS row stride: conflict-free column walks
 
 // Right-looking LU with partial pivoting of one matrix by one workgroup
@@ -326,21 +326,21 @@ __device__ void lu_matrix(float* A, long long lda, int n, int* ipiv, int* info)
         const int c1 = k + kb;
         const int mr = n - c1;
         if (mr <= 0) continue;
-        // 3. U12 := L11^{-1} A12 (unit lower L11 = pnl rows 0..kb)
-        for (int c = c1 + tid; c < n; c += 256) {
-            float x[kNB];
-#pragma unroll
-            for (int i = 0; i < kNB; ++i) x[i] = i < kb ? A[off(k + i, c, lda)] : 0.0f;
-#pragma unroll
-            for (int i = 1; i < kNB; ++i) {
-                float v = x[i];
-#pragma unroll
-                for (int q = 0; q < i; ++q) v -= pnl[i][q] * x[q];
-                x[i] = v;
+        // 3. U12 := L11^{-1} A12 (unit lower L11 = pnl rows 0..kb), one thread
+        //    per column; the column sits in LDS row kNB + tid of the panel
+        //    array (free once the panel is written back)
+        for (int c0 = c1; c0 < n; c0 += 256) {
+            const int c = c0 + tid;
+            float* x = pnl[kNB + tid];
+            if (c < n) {
+                for (int i = 0; i < kb; ++i) x[i] = A[off(k + i, c, lda)];
+                for (int i = 1; i < kb; ++i) {
+                    float v = x[i];
+                    for (int q = 0; q < i; ++q) v -= pnl[i][q] * x[q];
+                    x[i] = v;
+                }
+                for (int i = 0; i < kb; ++i) A[off(k + i, c, lda)] = x[i];
             }
-#pragma unroll
-            for (int i = 0; i < kNB; ++i)
-                if (i < kb) A[off(k + i, c, lda)] = x[i];
         }
         __syncthreads();
         // 4. A22 -= L21 U12

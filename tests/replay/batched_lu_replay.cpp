@@ -206,17 +206,19 @@ void lu_matrix(Checked& M, int n, int* info)
         }
         const int c1 = k + kb, mr = n - c1;
         if (mr <= 0) continue;
-        // 3. U12 := L11^{-1} A12
-        for (int c = c1; c < n; ++c) {
-            float x[kNB];
-            for (int ii = 0; ii < kNB; ++ii) x[ii] = ii < kb ? M.A(k + ii, c) : 0.0f;
-            for (int ii = 1; ii < kNB; ++ii) {
-                float v = x[ii];
-                for (int q = 0; q < ii; ++q) v -= pnl.at(ii, q) * x[q];
-                x[ii] = v;
+        // 3. U12 := L11^{-1} A12, thread tid's column in LDS row kNB + tid
+        for (int c0 = c1; c0 < n; c0 += kThreads)
+            for (int tid = 0; tid < kThreads; ++tid) {
+                const int c = c0 + tid;
+                if (c >= n) continue;
+                for (int ii = 0; ii < kb; ++ii) pnl.at(kNB + tid, ii) = M.A(k + ii, c);
+                for (int ii = 1; ii < kb; ++ii) {
+                    float v = pnl.at(kNB + tid, ii);
+                    for (int q = 0; q < ii; ++q) v -= pnl.at(ii, q) * pnl.at(kNB + tid, q);
+                    pnl.at(kNB + tid, ii) = v;
+                }
+                for (int ii = 0; ii < kb; ++ii) M.A(k + ii, c) = pnl.at(kNB + tid, ii);
             }
-            for (int ii = 0; ii < kb; ++ii) M.A(k + ii, c) = x[ii];
-        }
         // 4. A22 -= L21 U12
         const int tiles = (mr + kT - 1) / kT;
         for (int tj = 0; tj < tiles; ++tj)
