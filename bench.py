@@ -108,6 +108,18 @@ def block_sizes(args, n):
     return pnb, pb, gnb, gb, tb
 
 
+def gemm_policies(n):
+    """Update-tile policy (mifact_set_gemm_policy) per routine from the
+    autotuned table (0, the library default, when none is tuned)."""
+    try:
+        from hygondoubleblockedfactorizations_amd import tuning
+        t = tuning.lookup(n)
+        return {"potrf": int(t.get("potrf_gemm_policy", 0)),
+                "getrf": int(t.get("getrf_gemm_policy", 0))}
+    except Exception:
+        return {"potrf": 0, "getrf": 0}
+
+
 def gen_spd(torch, n, seed):
     """A = G G^T / n + I in FP32 on the GPU (exactly symmetric)."""
     g = torch.Generator(device="cuda")
@@ -550,7 +562,10 @@ def main():
     info = torch.zeros(1, dtype=torch.int32, device="cuda")
     ipiv = torch.zeros(n, dtype=torch.int32, device="cuda")
 
+    policy = gemm_policies(n)
+
     def run(routine):
+        lib.mifact_set_gemm_policy(h.ptr, policy[routine])
         if routine == "potrf":
             lib.double_blocked_potrf(h.ptr, P(work), n, n, pnb, pb, P(info))
         else:
@@ -674,11 +689,13 @@ def main():
                     t_gen = time.perf_counter()
                     A = spd_kappa(torch, n, args.solve_kappa, args.seed + 7)
                     log(f"kappa={args.solve_kappa:g} SPD matrix in {time.perf_counter() - t_gen:.1f} s")
+                    lib.mifact_set_gemm_policy(h.ptr, policy[r])
                     solve[r] = solve_compare(torch, lib, h, P, n, r, A, pnb, pb, tb,
                                              args.solve_precision, args.seed + 8, args.mixed_b)
                     solve[r]["matrix"] = f"Q diag(sigma) Q^T, kappa={args.solve_kappa:g}"
                     del A
                 else:
+                    lib.mifact_set_gemm_policy(h.ptr, policy[r])
                     solve[r] = solve_compare(torch, lib, h, P, n, r, inputs[r], gnb, gb, tb,
                                              args.solve_precision, args.seed + 9, args.mixed_b)
                     solve[r]["matrix"] = "iid Uniform(-1,1)"
@@ -717,7 +734,8 @@ def main():
         "config": {"workload": f"double-blocked POTRF + GETRF (partial pivoting), FP32, "
                                f"n={n}, column-major, lda=n",
                    "n": n, "potrf_nb": pnb, "potrf_b": pb, "getrf_nb": gnb, "getrf_b": gb,
-                   "getrf_outer_trsm": "iterative", "trsm_block": tb},
+                   "getrf_outer_trsm": "iterative", "trsm_block": tb,
+                   "gemm_policy": policy},
         "routines": per,
         "roofline": roofline,
         "cpu_baseline": cpu,

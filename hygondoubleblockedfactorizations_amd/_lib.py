@@ -26,6 +26,7 @@ SIGNATURES = {
     "mifact_get_last_error": [P],
     "mifact_reserve_workspace": [P, I],
     "mifact_set_gemm_config": [P, I],
+    "mifact_set_gemm_policy": [P, I],
     "mifact_set_wait_limit": [P, ctypes.c_longlong],
     "mifact_version": [],
     "mifact_profile_begin": [P],
@@ -118,6 +119,7 @@ _RESTYPE = {
     "mifact_get_last_error": I,
     "mifact_reserve_workspace": I,
     "mifact_set_gemm_config": I,
+    "mifact_set_gemm_policy": I,
     "mifact_set_wait_limit": I,
     "mifact_version": ctypes.c_char_p,
     "mifact_profile_begin": I,

@@ -46,6 +46,8 @@ struct mifact_handle_s {
     int error_mode = mifact_error_exit;
     // FP32 GEMM tile configuration (0 = automatic; see mifact_set_gemm_config).
     int gemm_config = 0;
+    // Automatic tile-selection policy (see mifact_set_gemm_policy).
+    int gemm_policy = 0;
     mifact_status last_error = mifact_status_success;
 
     // Device workspace arena: [ints | floats] sized for order ws_n.

@@ -971,8 +971,11 @@ static void dispatch(mifact_handle h, GemmOp op, GemmShape shape, bool beta0, in
         if (beta0) {
             cfg = 3;
         } else if (cfg == 0) {
-            if (shape == kShapeFull && tiles(256) >= 4LL * h->num_cu) cfg = 4;
-            else if (M > 64 && N > 64 && tiles(128) >= 2LL * h->num_cu) cfg = 1;
+            const int pol = h->gemm_policy;
+            const bool big_ok = pol != 1 && (shape == kShapeFull || pol == 2);
+            const long long mid = pol == 3 ? h->num_cu : 2LL * h->num_cu;
+            if (big_ok && tiles(256) >= 4LL * h->num_cu) cfg = 4;
+            else if (M > 64 && N > 64 && tiles(128) >= mid) cfg = 1;
             else cfg = 3;
         }
         switch (cfg) {

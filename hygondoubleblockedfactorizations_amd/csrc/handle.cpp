@@ -339,6 +339,14 @@ mifact_status mifact_set_gemm_config(mifact_handle h, int config)
     return mifact_status_success;
 }
 
+mifact_status mifact_set_gemm_policy(mifact_handle h, int policy)
+{
+    if (h == nullptr) return mifact_status_invalid_handle;
+    if (policy < 0 || policy > 3) return mifact_status_invalid_value;
+    h->gemm_policy = policy;
+    return mifact_status_success;
+}
+
 mifact_status mifact_set_wait_limit(mifact_handle h, long long polls)
 {
     if (h == nullptr) return mifact_status_invalid_handle;

@@ -1,10 +1,12 @@
-"""Autotuner for the double-blocked drivers' block sizes.
+"""Autotuner for the double-blocked drivers' block sizes and update tiles.
 
 The reference picks nb/b by hand per run (its benchmark scripts pass them on
 the command line; potrf_double_blocked_cases.cpp:12-114,
 getrf_double_blocked_cases.cpp:13-109).  Here the outer/inner block sizes of
-POTRF and GETRF and the GETRF outer-TRSM block are measured on the device:
-every candidate of a small grid is timed (HIP events, median of `reps` runs
+POTRF and GETRF, the GETRF outer-TRSM block and, at the chosen blocks, the
+MFMA tile-selection policy of the update GEMMs (mifact_set_gemm_policy:
+which updates take 256 x 256, 128 x 128 or 64 x 64 tiles) are measured on
+the device: every candidate is timed (HIP events, median of `reps` runs
 after one warm-up) on the same seeded inputs as the benchmark, and the
 fastest is stored in ``tuning_table.json`` next to this file, keyed by n.
 ``lookup(n)`` returns the entry of the nearest tuned order (block sizes
@@ -28,6 +30,7 @@ POTRF_GRID = [(2048, 256), (4096, 256), (4096, 512), (8192, 256), (8192, 512), (
 GETRF_GRID = [(4096, 2048), (4096, 4096), (6144, 3072), (8192, 2048), (8192, 4096),
               (16384, 4096)]
 TRSM_BLOCKS = [2048, 4096]
+GEMM_POLICIES = [0, 1, 2, 3]   # mifact_set_gemm_policy
 
 
 def defaults(n: int) -> dict:
@@ -35,7 +38,7 @@ def defaults(n: int) -> dict:
     return {
         "potrf_nb": min(8192, n), "potrf_b": min(1024, n),
         "getrf_nb": min(max(n // 2, 1), 20480), "getrf_b": min(max(n // 4, 1), 4096),
-        "trsm_block": 2048,
+        "trsm_block": 2048, "potrf_gemm_policy": 0, "getrf_gemm_policy": 0,
     }
 
 
@@ -57,7 +60,10 @@ def lookup(n: int, path: str = TABLE) -> dict:
         e[k] = max(1, min(int(e[k]), n))
     e["potrf_b"] = min(e["potrf_b"], e["potrf_nb"])
     e["getrf_b"] = min(e["getrf_b"], e["getrf_nb"])
-    return {k: e[k] for k in ("potrf_nb", "potrf_b", "getrf_nb", "getrf_b", "trsm_block")}
+    e.setdefault("potrf_gemm_policy", 0)
+    e.setdefault("getrf_gemm_policy", 0)
+    return {k: e[k] for k in ("potrf_nb", "potrf_b", "getrf_nb", "getrf_b", "trsm_block",
+                              "potrf_gemm_policy", "getrf_gemm_policy")}
 
 
 def _time(torch, fn, reset, reps):
@@ -123,6 +129,27 @@ def tune(n: int, reps: int = 2, log=print) -> dict:
     else:
         d = defaults(n)
         best.update(getrf_nb=d["getrf_nb"], getrf_b=d["getrf_b"], trsm_block=d["trsm_block"])
+    # update-tile policy at the chosen blocks
+    for routine in ("potrf", "getrf"):
+        res = []
+        for pol in GEMM_POLICIES:
+            lib.mifact_set_gemm_policy(h.ptr, pol)
+            if routine == "potrf":
+                nb, b = best["potrf_nb"], best["potrf_b"]
+                ms = _time(torch, lambda: lib.double_blocked_potrf(h.ptr, P(work), n, n, nb, b,
+                                                                   P(info)),
+                           lambda: work.copy_(spd), reps)
+            else:
+                nb, b, tb = best["getrf_nb"], best["getrf_b"], best["trsm_block"]
+                ms = _time(torch, lambda: lib.double_blocked_getrf(h.ptr, P(work), n, n, nb, b, 1,
+                                                                   tb, P(ipiv), P(info)),
+                           lambda: work.copy_(gen), reps)
+            res.append((ms, pol))
+            log(f"  {routine} n={n} gemm policy {pol}: {ms:.2f} ms")
+        lib.mifact_set_gemm_policy(h.ptr, 0)
+        ms, pol = min(res)
+        best[f"{routine}_gemm_policy"] = pol
+        best[f"{routine}_ms"] = ms
     if h.last_error() != 0:
         raise RuntimeError("library reported an error while tuning")
     return best

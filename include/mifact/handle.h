@@ -76,6 +76,13 @@ mifact_status mifact_reserve_workspace(mifact_handle handle, int n);
  * / 16-deep, 4 = 256x256 / 16-deep on eight waves. */
 mifact_status mifact_set_gemm_config(mifact_handle handle, int config);
 
+/* Rule by which the automatic configuration (config 0) picks FP32 update
+ * tiles, the autotuner's choice per routine and order: 0 (default) = 256 x
+ * 256 for full-shape updates with >= 4 tiles per CU, 128 x 128 with >= 2,
+ * 64 x 64 below; 1 = never 256 x 256; 2 = 256 x 256 also for lower-trapezoid
+ * updates; 3 = as 0 but 128 x 128 from 1 tile per CU. */
+mifact_status mifact_set_gemm_policy(mifact_handle handle, int policy);
+
 /* Library version string. */
 const char* mifact_version(void);
 
