@@ -182,52 +182,140 @@ __device__ __attribute__((always_inline)) inline void inverse_from_lds(int s,
 }
 
 // ------------------------------------------------------------------ POTF2
-// Unblocked lower Cholesky of an s x s block (s <= 64) by one wave64: lane
-// i holds row i in registers (rows beyond s padded with the identity);
-// per column the pivot is a square root and the column is scaled by its
-// reciprocal (LAPACK xPOTF2 arithmetic), then every lane updates its row
-// (right-looking).  A non-positive or NaN pivot records col_offset + j + 1
-// in *info (first failure in launch order), leaves the columns before it
-// factored and everything from it on unchanged.  When inv is not null the
-// wave then writes inv(L) (column-major, ld 64): the panel TRSM applies it
-// with an MFMA multiply instead of a substitution.
+// Rows o .. o + N - 1 of inv(L_b), L_b = L(o:o+N, o:o+N) the lower block in
+// sL (1 / diagonal in rd): lane i < N holds row i, m[k] = M(i, k), from the
+// last column to the first: M(i, k) = (delta_ik - sum_{k' > k} M(i, k')
+// L(k', k)) rd[k] (M L = I; the L entries are LDS broadcasts).
+template <typename T, int N>
+__device__ __attribute__((always_inline)) inline void lower_inverse_rows(const T (*sL)[kLdsLd],
+                                                                          const T* rd, int o,
+                                                                          int i, T (&m)[N])
+{
+#pragma unroll
+    for (int k = N - 1; k >= 0; --k) {
+        T acc = (i == k) ? T(1) : T(0);
+#pragma unroll
+        for (int kp = k + 1; kp < N; ++kp) acc -= m[kp] * sL[o + kp][kPadL + o + k];
+        m[k] = acc * rd[o + k];
+        // one row entry at a time: the broadcasts are not all hoisted
+        asm volatile("" : "+v"(m[k]));
+    }
+}
+
+// Unblocked lower Cholesky of an s x s block (s <= 64).  The block is
+// loaded into LDS by four waves; wave 0 factors it with lane i holding row i
+// in registers (rows beyond s padded with the identity): per column the
+// pivot is a square root and the column is scaled by its reciprocal (LAPACK
+// xPOTF2 arithmetic), then every lane updates its row (right-looking).  A
+// non-positive or NaN pivot records col_offset + j + 1 in *info (first
+// failure in launch order), leaves the columns before it factored and
+// everything from it on unchanged.  When inv is not null the four waves then
+// write inv(L) (column-major, ld 64) as 2 x 2 blocks of 32: inv(L11) and
+// inv(L22) by waves 0 and 1 side by side, then -inv(L22) L21 inv(L11) as two
+// 32 x 32 products over all threads.  The panel TRSM applies inv(L) with an
+// MFMA multiply instead of a substitution.
 template <typename T>
-__global__ __launch_bounds__(64) void potf2_kernel(int s, T* __restrict__ A, long long lda,
-                                                   T* __restrict__ inv, int* __restrict__ info,
-                                                   int col_offset)
+__global__ __launch_bounds__(256) void potf2_kernel(int s, T* __restrict__ A, long long lda,
+                                                    T* __restrict__ inv, int* __restrict__ info,
+                                                    int col_offset)
 {
     __shared__ __attribute__((aligned(16))) T col[2 * kLeaf];
     __shared__ __attribute__((aligned(16))) T sL[kLeaf][kLdsLd];
     __shared__ T rd[kLeaf];
-    const int i = threadIdx.x;
-    T x[kLeaf];
-    load_row(A, lda, s, i < s ? i : s - 1, x);
+    __shared__ T mb[3][32][33];   // inv(L11), inv(L22), L21 inv(L11)
+    __shared__ int jf_s;
+    const int tid = threadIdx.x;
+    const int i = tid & 63;
+    const int wv = tid >> 6;
+    // the block (lower part, identity beyond s) -> LDS; wave wv: columns
+    // 16 wv .. 16 wv + 15 (coalesced down each column) and its share of the
+    // zero padding in front of each row
 #pragma unroll
-    for (int t = 0; t < kLeaf; ++t)
-        x[t] = (i < s) ? (t <= i ? x[t] : T(0)) : (t == i ? T(1) : T(0));
-    col[kLeaf + i] = T(0);
+    for (int t = 0; t < 16; ++t) {
+        const int c = 16 * wv + t;
+        T v;
+        if (i < s) v = (c <= i) ? A[off(i, c, lda)] : T(0);
+        else v = (c == i) ? T(1) : T(0);
+        sL[i][kPadL + c] = v;
+    }
 #pragma unroll
-    for (int t = 0; t < kPadL; t += 4)
-        *reinterpret_cast<Vec4<T>*>(&sL[i][t]) = Vec4<T>{T(0), T(0), T(0), T(0)};
-    int jf = kLeaf;
-    const T d0 = lane_value(x[0], 0);
-    if (!(d0 > T(0))) jf = 0;
-    T ljj = sqrt(d0);
-    T r = T(1) / ljj;
-    chol_phase<T, 64>(0, 16, x, col, sL, rd, jf, ljj, r);
-    chol_phase<T, 48>(16, 32, x, col, sL, rd, jf, ljj, r);
-    chol_phase<T, 32>(32, 48, x, col, sL, rd, jf, ljj, r);
-    chol_phase<T, 16>(48, kLeaf, x, col, sL, rd, jf, ljj, r);
-    // row i's finished columns (all, or those before a failing pivot)
+    for (int t = 0; t < 16; t += 4)
+        *reinterpret_cast<Vec4<T>*>(&sL[i][16 * wv + t]) = Vec4<T>{T(0), T(0), T(0), T(0)};
+    if (tid < kLeaf) col[kLeaf + tid] = T(0);
+    __syncthreads();
+    if (wv == 0) {
+        T x[kLeaf];
+#pragma unroll
+        for (int t = 0; t < kLeaf; t += 4) {
+            const Vec4<T> v = *reinterpret_cast<const Vec4<T>*>(&sL[i][kPadL + t]);
+            x[t] = v[0];
+            x[t + 1] = v[1];
+            x[t + 2] = v[2];
+            x[t + 3] = v[3];
+        }
+        int jf = kLeaf;
+        const T d0 = lane_value(x[0], 0);
+        if (!(d0 > T(0))) jf = 0;
+        T ljj = sqrt(d0);
+        T r = T(1) / ljj;
+        chol_phase<T, 64>(0, 16, x, col, sL, rd, jf, ljj, r);
+        chol_phase<T, 48>(16, 32, x, col, sL, rd, jf, ljj, r);
+        chol_phase<T, 32>(32, 48, x, col, sL, rd, jf, ljj, r);
+        chol_phase<T, 16>(48, kLeaf, x, col, sL, rd, jf, ljj, r);
+        if (i == 0) jf_s = jf;
+    }
+    __syncthreads();
+    const int jf = jf_s;
+    // the finished columns (all, or those before a failing pivot)
     const int done = jf < s ? jf : s;
-    if (i < s)
-        for (int t = 0; t <= i && t < done; ++t) A[off(i, t, lda)] = sL[i][kPadL + t];
+    if (i < s) {
+#pragma unroll
+        for (int t = 0; t < 16; ++t) {
+            const int c = 16 * wv + t;
+            if (c <= i && c < done) A[off(i, c, lda)] = sL[i][kPadL + c];
+        }
+    }
     if (jf < s) {
-        if (i == 0 && info != nullptr) atomicCAS(info, 0, col_offset + jf + 1);
+        if (tid == 0 && info != nullptr) atomicCAS(info, 0, col_offset + jf + 1);
         return;
     }
     if (inv == nullptr) return;
-    inverse_from_lds<T>(s, sL, rd, inv);
+    if (wv < 2) {
+        T m[32];
+        lower_inverse_rows<T, 32>(sL, rd, 32 * wv, i, m);
+        if (i < 32) {
+#pragma unroll
+            for (int k = 0; k < 32; ++k) mb[wv][i][k] = m[k];
+        }
+    }
+    __syncthreads();
+    // thread: row r, columns c0 .. c0 + 3 of the 32 x 32 off-diagonal products
+    const int r = tid & 31;
+    const int c0 = (tid >> 5) * 4;
+    T acc[4] = {T(0), T(0), T(0), T(0)};
+    for (int k = 0; k < 32; ++k) {
+        const T l = sL[32 + r][kPadL + k];
+#pragma unroll
+        for (int e = 0; e < 4; ++e) acc[e] += l * mb[0][k][c0 + e];
+    }
+#pragma unroll
+    for (int e = 0; e < 4; ++e) mb[2][r][c0 + e] = acc[e];
+    __syncthreads();
+    T m21[4] = {T(0), T(0), T(0), T(0)};
+    for (int k = 0; k < 32; ++k) {
+        const T g = mb[1][r][k];
+#pragma unroll
+        for (int e = 0; e < 4; ++e) m21[e] -= g * mb[2][k][c0 + e];
+    }
+    // inv (ld 64), entries (row, column) with both below s
+#pragma unroll
+    for (int e = 0; e < 4; ++e) {
+        const int c = c0 + e;
+        if (32 + r < s && c < s) inv[c * kLeaf + 32 + r] = m21[e];
+        if (r < s && c < s) inv[c * kLeaf + r] = mb[0][r][c];
+        if (r < s && 32 + c < s) inv[(32 + c) * kLeaf + r] = T(0);
+        if (32 + r < s && 32 + c < s) inv[(32 + c) * kLeaf + 32 + r] = mb[1][r][c];
+    }
 }
 
 // Unpivoted LU of an s x s block (s <= 64) in one wave: lane i owns row i
@@ -282,7 +370,7 @@ void potf2_base(mifact_handle h, int s, T* A, long long lda, T* inv, int* info, 
     }
     StageScope scope(h, kStagePotf2, static_cast<double>(s) * s * s / 3.0,
                      sizeof(T) * static_cast<double>(s) * s * 1.5);
-    hipLaunchKernelGGL(potf2_kernel<T>, dim3(1), dim3(64), 0, h->stream, s, A, lda, inv,
+    hipLaunchKernelGGL(potf2_kernel<T>, dim3(1), dim3(256), 0, h->stream, s, A, lda, inv,
                        info, col_offset);
     MIFACT_LAUNCHED(h);
 }
