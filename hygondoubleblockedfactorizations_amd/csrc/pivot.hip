@@ -172,7 +172,8 @@ struct Rec {
     static constexpr int TAG = PW * LW;         // the row's origin tag
     static constexpr int VAL = TAG + 1;         // |value| of the candidate
     static constexpr int IDX = VAL + PW;        // candidate row (label)
-    static constexpr int RW = IDX + 1;          // words per record
+    static constexpr int XCC = IDX + 1;         // the publisher's XCD
+    static constexpr int RW = XCC + 1;          // words per record
     static constexpr int PARITY = kMaxG * RW;
 };
 
@@ -198,6 +199,7 @@ struct LeafArgs {
     unsigned long long spin_limit;  // unsuccessful polls before giving up
     int G;                          // workgroups that hold rows
     int stride;                     // launched blocks per holding workgroup
+    int one_xcd;                    // launch shape puts every holding workgroup on one XCD
     T* inv;                         // inverse of the leaf's unit-lower block (ld 64) or null
     T* inv_zero;                    // 32 x 32 block (ld 64) to clear, or null
     int* mv_src;                    // move list of the leaf's pivots (2w entries) or null
@@ -208,6 +210,23 @@ __device__ inline void xstore64(unsigned long long* p, unsigned int payload, uns
 {
     const unsigned long long v = (static_cast<unsigned long long>(epoch) << 32) | payload;
     __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// The same word for readers on the writer's own XCD only: a plain store that
+// stays in the XCD's L2, which the readers' device-scope loads hit (about
+// half the latency of a store written through to memory).
+__device__ inline void xstore64_xcd(unsigned long long* p, unsigned int payload, unsigned int epoch)
+{
+    const unsigned long long v = (static_cast<unsigned long long>(epoch) << 32) | payload;
+    __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+}
+
+// XCD of the executing workgroup (hardware register XCC_ID, bits 3:0).
+__device__ inline unsigned int xcd_id()
+{
+    unsigned int v;
+    asm volatile("s_getreg_b32 %0, hwreg(HW_REG_XCC_ID)" : "=s"(v));
+    return v & 0xfu;
 }
 
 __device__ inline unsigned long long xload64(const unsigned long long* p)
@@ -279,7 +298,7 @@ __device__ inline int leaf_index(int W, int c)
 template <typename T, int RPT, int NP, int W, int U>
 __device__ __attribute__((always_inline)) inline bool leaf_step(
     const LeafArgs<T>& a, LeafShared<T>& sh, T (&x)[RPT][kLeafWidth<T>], int (&tag)[RPT],
-    int (&lab)[RPT], T (&lp)[RPT], int base, int g)
+    int (&lab)[RPT], T (&lp)[RPT], bool& local, int base, int g)
 {
     using R = Rec<T>;
     constexpr int LW = kLeafWidth<T>;
@@ -339,7 +358,9 @@ __device__ __attribute__((always_inline)) inline bool leaf_step(
     //    multiply-add its owner applies below)
     if (tid < R::RW) {
         unsigned int word;
-        if (wi == kIdxNone) {
+        if (tid == R::XCC) {
+            word = xcd_id();
+        } else if (wi == kIdxNone) {
             word = (tid == R::TAG) ? 0xffffffffu
                  : (tid >= R::VAL && tid < R::IDX) ? word_of(T(-1), tid - R::VAL)
                  : (tid == R::IDX) ? static_cast<unsigned int>(kIdxNone) : 0u;
@@ -356,7 +377,8 @@ __device__ __attribute__((always_inline)) inline bool leaf_step(
         } else {
             word = static_cast<unsigned int>(wi);
         }
-        xstore64(slot + static_cast<size_t>(g) * R::RW + tid, word, epoch);
+        if (local) xstore64_xcd(slot + static_cast<size_t>(g) * R::RW + tid, word, epoch);
+        else xstore64(slot + static_cast<size_t>(g) * R::RW + tid, word, epoch);
     }
 
     // 4. deferred update of column j - 1 on own rows, columns after j
@@ -416,6 +438,14 @@ __device__ __attribute__((always_inline)) inline bool leaf_step(
             atomicCAS(a.info, 0, -2);
         }
         return false;
+    }
+    // after the leaf's first column: if every record came from this
+    // workgroup's XCD, later columns publish with XCD-local stores
+    if (U == 0 && j == 0 && a.one_xcd) {
+        const unsigned int x0 = sh.rec[0][R::XCC];
+        bool same = true;
+        for (int t = lane; t < G; t += 64) same = same && sh.rec[t][R::XCC] == x0;
+        local = __all(same);
     }
 
     // 6. global pivot, computed by every wave from the gathered records (lane
@@ -510,18 +540,18 @@ __device__ __attribute__((always_inline)) inline void leaf_rotate(T (&x)[RPT][kL
 template <typename T, int RPT, int NP, int W>
 __device__ __attribute__((always_inline)) inline bool leaf_phase(
     const LeafArgs<T>& a, LeafShared<T>& sh, T (&x)[RPT][kLeafWidth<T>], int (&tag)[RPT],
-    int (&lab)[RPT], T (&lp)[RPT], int g)
+    int (&lab)[RPT], T (&lp)[RPT], bool& local, int g)
 {
     constexpr int LW = kLeafWidth<T>;
     constexpr int QP = LW - W;
     for (int base = QP; base < QP + kLeafQ; base += kLeafU) {
-        if (base < a.w && !leaf_step<T, RPT, NP, W, 0>(a, sh, x, tag, lab, lp, base, g))
+        if (base < a.w && !leaf_step<T, RPT, NP, W, 0>(a, sh, x, tag, lab, lp, local, base, g))
             return false;
-        if (base + 1 < a.w && !leaf_step<T, RPT, NP, W, 1>(a, sh, x, tag, lab, lp, base, g))
+        if (base + 1 < a.w && !leaf_step<T, RPT, NP, W, 1>(a, sh, x, tag, lab, lp, local, base, g))
             return false;
-        if (base + 2 < a.w && !leaf_step<T, RPT, NP, W, 2>(a, sh, x, tag, lab, lp, base, g))
+        if (base + 2 < a.w && !leaf_step<T, RPT, NP, W, 2>(a, sh, x, tag, lab, lp, local, base, g))
             return false;
-        if (base + 3 < a.w && !leaf_step<T, RPT, NP, W, 3>(a, sh, x, tag, lab, lp, base, g))
+        if (base + 3 < a.w && !leaf_step<T, RPT, NP, W, 3>(a, sh, x, tag, lab, lp, local, base, g))
             return false;
         leaf_rotate<T, RPT, W>(x);
     }
@@ -589,10 +619,13 @@ __global__ __launch_bounds__(kLeafThreads) void getf2_leaf_kernel(LeafArgs<T> a)
         lp[r] = T(0);
     }
 
-    bool ok = leaf_phase<T, RPT, NP, LW>(a, sh, x, tag, lab, lp, g);
-    if constexpr (LW > kLeafQ) ok = ok && leaf_phase<T, RPT, NP, LW - kLeafQ>(a, sh, x, tag, lab, lp, g);
-    if constexpr (LW > 2 * kLeafQ) ok = ok && leaf_phase<T, RPT, NP, LW - 2 * kLeafQ>(a, sh, x, tag, lab, lp, g);
-    if constexpr (LW > 3 * kLeafQ) ok = ok && leaf_phase<T, RPT, NP, LW - 3 * kLeafQ>(a, sh, x, tag, lab, lp, g);
+    // the first column publishes with device-scope stores (and its XCD);
+    // later ones with XCD-local stores if every record came from one XCD
+    bool local = false;
+    bool ok = leaf_phase<T, RPT, NP, LW>(a, sh, x, tag, lab, lp, local, g);
+    if constexpr (LW > kLeafQ) ok = ok && leaf_phase<T, RPT, NP, LW - kLeafQ>(a, sh, x, tag, lab, lp, local, g);
+    if constexpr (LW > 2 * kLeafQ) ok = ok && leaf_phase<T, RPT, NP, LW - 2 * kLeafQ>(a, sh, x, tag, lab, lp, local, g);
+    if constexpr (LW > 3 * kLeafQ) ok = ok && leaf_phase<T, RPT, NP, LW - 3 * kLeafQ>(a, sh, x, tag, lab, lp, local, g);
     static_assert(LW <= 4 * kLeafQ && LW % kLeafQ == 0, "leaf of one to four phases");
     if (!ok) return;
 
@@ -732,6 +765,7 @@ void getf2_leaf(mifact_handle h, T* A, long long lda, int n, int c0, int w, int*
     int stride = 8;
     while (stride > 1 && args.G * stride > h->num_cu) stride /= 2;
     args.stride = stride;
+    args.one_xcd = stride == 8;
     launch_leaf_np<T, 1>(h, args);
 }
 
