@@ -79,6 +79,11 @@ def parse():
     ap.add_argument("--solve-kappa", type=float, default=1e4,
                     help="condition number of the SPD solve matrix Q diag(sigma) Q^T")
     ap.add_argument("--no-batched", action="store_true", help="skip the batched section")
+    ap.add_argument("--solve-sweep", action="store_true",
+                    help="run only the full-size precision sweep (POTRF kappa x precision x "
+                         "nrhs, GETRF uniform and kappa=1e5 x FP16/FP8) and print it as JSON")
+    ap.add_argument("--sweep-potrf-n", type=int, default=32768)
+    ap.add_argument("--sweep-getrf-n", default="32768,49152")
     ap.add_argument("--batch-spd", type=int, default=4096)
     ap.add_argument("--batch-lu", type=int, default=2048)
     return ap.parse_args()
@@ -339,6 +344,230 @@ def solve_compare(torch, lib, h, P, n, routine, A, nb, b, tb, prec, seed, mixed_
     return out
 
 
+# torch's dense FP64 factorizations (QR, LU, Cholesky) return wrong results
+# for matrices of 2^31 elements or more (n = 49152 gave a singular "Q" and a
+# wrong LU solve); above this size the sweep builds its matrices and FP64
+# reference solutions from blocks that stay below it.
+_TORCH_DENSE_MAX = 2 ** 31 - 1
+
+
+def orthogonal(torch, n, seed):
+    """Orthogonal factor of a seeded Gaussian n x n matrix (FP64, GPU): the Q
+    of its Householder QR; for n^2 >= 2^31, block classical Gram-Schmidt
+    with reorthogonalisation over 2048-column blocks (the same Q up to the
+    signs of its columns)."""
+    g = torch.Generator(device="cuda")
+    g.manual_seed(seed)
+    X = torch.randn(n, n, dtype=torch.float64, device="cuda", generator=g)
+    if n * n <= _TORCH_DENSE_MAX:
+        q, _ = torch.linalg.qr(X)
+        return q
+    nb, cb = 2048, 8192
+    for j0 in range(0, n, nb):
+        j1 = min(n, j0 + nb)
+        blk = X[:, j0:j1]
+        for _ in range(2):                       # CGS2
+            for c0 in range(0, j0, cb):
+                c1 = min(j0, c0 + cb)
+                Qc = X[:, c0:c1]
+                blk -= Qc @ (Qc.t() @ blk)
+        q, _ = torch.linalg.qr(blk)
+        X[:, j0:j1] = q
+    return X
+
+
+def _block_lu_solve(torch, M, B):
+    """FP64 solve through the 2 x 2 block partition of M (for n^2 >= 2^31):
+    LAPACK LU (torch) of M11 and of the Schur complement
+    S = M22 - M21 M11^{-1} M12, block substitution, then one refinement step
+    with the FP64 residual of the whole matrix."""
+    n = M.shape[0]
+    h = n // 2
+    M11, M12, M21, M22 = M[:h, :h], M[:h, h:], M[h:, :h], M[h:, h:]
+    F11 = torch.linalg.lu_factor(M11)
+    W = torch.linalg.lu_solve(*F11, M12)          # M11^{-1} M12
+    S = M22 - M21 @ W
+    FS = torch.linalg.lu_factor(S)
+    del S
+
+    def solve(R):
+        y1 = torch.linalg.lu_solve(*F11, R[:h])
+        x2 = torch.linalg.lu_solve(*FS, R[h:] - M21 @ y1)
+        return torch.cat([y1 - W @ x2, x2])
+
+    X = solve(B)
+    X += solve(B - M @ X)
+    del W, F11, FS
+    return X
+
+
+def fp64_reference(torch, routine, M, B):
+    """X = M^{-1} B by FP64 LAPACK factorizations on the GPU (torch's
+    Cholesky / LU: dpotrf+dpotrs, dgetrf+dgetrs; block partition above 2^31
+    elements).  M is the FP64 matrix in natural (row-major) orientation; B
+    and X are n x nrhs."""
+    n = M.shape[0]
+    if n * n > _TORCH_DENSE_MAX:
+        return _block_lu_solve(torch, M, B)
+    if routine == "potrf":
+        L = torch.linalg.cholesky(M)
+        X = torch.cholesky_solve(B, L)
+        del L
+    else:
+        LU, piv = torch.linalg.lu_factor(M)
+        X = torch.linalg.lu_solve(LU, piv, B)
+        del LU, piv
+    return X
+
+
+def sweep_case(torch, lib, h, P, routine, A, M64, nrhs, precisions, blocks, mixed_b, seed):
+    """One matrix and right-hand-side count of the solve sweep: the plain
+    FP32 solve (factorization + triangular solves) and, per precision, the
+    mixed factorization + FP64 refinement to the FP32 solve's backward
+    error.  Errors: normwise backward error max_j ||b_j - A x_j|| /
+    (||A|| ||x_j|| + ||b_j||) (inf-norms, FP64 residual) and forward error
+    ||x - x_ref|| / ||x_ref|| against the FP64 LAPACK solution x_ref (and
+    against x_true).  A: FP32 column-major buffer; M64: the same matrix in
+    FP64, natural orientation."""
+    import ctypes
+    from hygondoubleblockedfactorizations_amd import _lib
+    n = M64.shape[0]
+    nb, b, tb = blocks
+    g = torch.Generator(device="cuda")
+    g.manual_seed(seed)
+    # right-hand sides as n x nrhs column-major buffers = (nrhs, n) tensors
+    Xt = torch.randn(nrhs, n, dtype=torch.float64, device="cuda", generator=g)
+    B = (M64 @ Xt.t()).t().contiguous()
+    t0 = time.perf_counter()
+    Xref = fp64_reference(torch, routine, M64, B.t().contiguous()).t().contiguous()
+    torch.cuda.synchronize()
+    t_ref = time.perf_counter() - t0
+    sym = 1 if routine == "potrf" else 0
+    anorm = A.abs().sum(0).max().item()      # buffer column sums = rows of the matrix
+    stream = torch.cuda.current_stream()
+    ev0, ev1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+
+    def errors(X):
+        R = torch.empty_like(B)
+        lib.mifact_residual_fp64(h.ptr, sym, n, nrhs, P(A), n, P(X), n, P(B), n, P(R), n)
+        torch.cuda.synchronize()
+        berr = (R.abs().amax(1) / (anorm * X.abs().amax(1) + B.abs().amax(1))).max().item()
+        fref = ((X - Xref).abs().amax(1) / Xref.abs().amax(1)).max().item()
+        ftrue = ((X - Xt).abs().amax(1) / Xt.abs().amax(1)).max().item()
+        return berr, fref, ftrue
+
+    work = torch.empty_like(A)
+    info = torch.zeros(1, dtype=torch.int32, device="cuda")
+    ipiv = torch.zeros(n, dtype=torch.int32, device="cuda")
+    for rep in range(2):
+        work.copy_(A)
+        X32 = B.float()
+        torch.cuda.synchronize()
+        ev0.record(stream)
+        if routine == "potrf":
+            lib.double_blocked_potrf(h.ptr, P(work), n, n, nb, b, P(info))
+            lib.mifact_potrs(h.ptr, n, nrhs, P(work), n, P(X32), n)
+        else:
+            lib.double_blocked_getrf(h.ptr, P(work), n, n, nb, b, 1, tb, P(ipiv), P(info))
+            lib.mifact_getrs(h.ptr, n, nrhs, P(work), n, P(ipiv), P(X32), n)
+        ev1.record(stream)
+        torch.cuda.synchronize()
+        t32 = ev0.elapsed_time(ev1)
+    berr32, fref32, ftrue32 = errors(X32.double())
+    case = {"n": n, "nrhs": nrhs, "fp64_reference_s": t_ref,
+            "fp32_solve": {"ms": t32, "backward_error": berr32, "forward_error_vs_fp64": fref32,
+                           "forward_error_vs_true": ftrue32, "info": int(info[0])},
+            "mixed": {}}
+    del X32
+    for prec in precisions:
+        X = torch.zeros_like(B)
+        prm = _lib.RefineParams(_lib.PRECISION[prec], nb, max(b, min(mixed_b, nb)), 1, tb, 30,
+                                berr32, 1, _lib.REFINE_METHOD["auto"])
+        res = _lib.RefineResult()
+        for rep in range(2):
+            X.zero_()
+            torch.cuda.synchronize()
+            ev0.record(stream)
+            if routine == "potrf":
+                st = lib.mifact_posv_refine(h.ptr, n, nrhs, P(A), n, P(work), n, P(B), n, P(X), n,
+                                            ctypes.byref(prm), ctypes.byref(res))
+            else:
+                st = lib.mifact_gesv_refine(h.ptr, n, nrhs, P(A), n, P(work), n, P(ipiv), P(B), n,
+                                            P(X), n, ctypes.byref(prm), ctypes.byref(res))
+            ev1.record(stream)
+            torch.cuda.synchronize()
+            if st != 0:
+                raise RuntimeError(f"refine status {st}")
+        tm = ev0.elapsed_time(ev1)
+        berr, fref, ftrue = errors(X)
+        case["mixed"][prec] = dict(res.as_dict(), ms=tm, backward_error_check=berr,
+                                   forward_error_vs_fp64=fref, forward_error_vs_true=ftrue,
+                                   tolerance=berr32, speedup_time_to_solution=t32 / tm)
+        log(f"  {routine} n={n} nrhs={nrhs} {prec}: {tm:.1f} ms (fp32 solve {t32:.1f}) "
+            f"berr {berr:.2e} (fp32 {berr32:.2e}) fwd/fp64 {fref:.2e} (fp32 {fref32:.2e}) "
+            f"steps {res.iterations} fgmres {res.gmres_iterations} fallback {res.fell_back}")
+    return case
+
+
+def solve_sweep(torch, lib, h, P, args):
+    """The precision pattern at full size.  POTRF: SPD A = Q diag(sigma)
+    Q^T (Q from the Householder QR of a seeded Gaussian, sigma geometric)
+    for kappa in {1e2, 1e4, 1e6}, BF16 and FP16 updates, nrhs in {1, 16}.
+    GETRF: iid U(-1, 1) and A = U diag(sigma) V^T (seeded orthogonal U, V,
+    kappa = 1e5) at each order of `--sweep-getrf-n`, FP16 and FP8 e4m3
+    updates, nrhs = 1.  Errors against FP64 LAPACK (dpotrs / dgetrs on the
+    GPU through torch)."""
+    out = {"potrf": [], "getrf": []}
+    seed = args.seed + 101
+    n = args.sweep_potrf_n
+    if n > 0:
+        pnb, pb, _, _, tb = block_sizes(args, n)
+        lib.mifact_set_gemm_policy(h.ptr, gemm_policies(n)["potrf"])
+        h.reserve(n)
+        Q = orthogonal(torch, n, seed)
+        for kappa in (1e2, 1e4, 1e6):
+            sig = kappa ** (-torch.arange(n, dtype=torch.float64, device="cuda") / max(n - 1, 1))
+            M = (Q * sig) @ Q.t()
+            M = 0.5 * (M + M.t())
+            A = M.float().contiguous()
+            M = A.double()          # the stored FP32 matrix, exactly
+            for nrhs in (1, 16):
+                log(f"sweep potrf n={n} kappa={kappa:g} nrhs={nrhs}")
+                case = sweep_case(torch, lib, h, P, "potrf", A, M, nrhs, ("bf16", "fp16"),
+                                  (pnb, pb, tb), args.mixed_b, seed + nrhs)
+                case["matrix"] = f"Q diag(sigma) Q^T, kappa={kappa:g}"
+                out["potrf"].append(case)
+            del A, M
+        del Q
+    for n in [int(v) for v in args.sweep_getrf_n.split(",") if v.strip()]:
+        _, _, gnb, gb, tb = block_sizes(args, n)
+        lib.mifact_set_gemm_policy(h.ptr, gemm_policies(n)["getrf"])
+        h.reserve(n)
+        for kind in ("uniform", "svd"):
+            if kind == "uniform":
+                g = torch.Generator(device="cuda")
+                g.manual_seed(seed + n)
+                A = torch.rand(n, n, dtype=torch.float32, device="cuda", generator=g).mul_(2).sub_(1)
+                label = "iid Uniform(-1,1)"
+            else:
+                U = orthogonal(torch, n, seed + n + 1)
+                V = orthogonal(torch, n, seed + n + 2)
+                sig = 1e5 ** (-torch.arange(n, dtype=torch.float64, device="cuda") / max(n - 1, 1))
+                # buffer (row-major tensor) = transpose of the matrix U S V^T
+                A = ((V * sig) @ U.t()).float().contiguous()
+                del U, V
+                label = "U diag(sigma) V^T, kappa=1e5"
+            M = A.t().double().contiguous()
+            log(f"sweep getrf n={n} {label}")
+            case = sweep_case(torch, lib, h, P, "getrf", A, M, 1, ("fp16", "fp8"),
+                              (gnb, gb, tb), args.mixed_b, seed + 7)
+            case["matrix"] = label
+            out["getrf"].append(case)
+            del A, M
+            torch.cuda.empty_cache()
+    return out
+
+
 def spd_kappa(torch, n, kappa, seed):
     """A = Q diag(sigma) Q^T, Q from the Householder QR of a seeded Gaussian
     (FP64 on the GPU), sigma_i = kappa^(-i/(n-1)); stored FP32."""
@@ -547,6 +776,12 @@ def main():
     h = _lib.Handle(stream=stream)
     h.reserve(n)
     P = _lib.ptr
+
+    if args.solve_sweep:
+        t0 = time.perf_counter()
+        sweep = solve_sweep(torch, lib, h, P, args)
+        print(json.dumps({"solve_sweep": sweep, "seconds": time.perf_counter() - t0}), flush=True)
+        return
 
     log(f"n={n} potrf nb/b={pnb}/{pb} getrf nb/b={gnb}/{gb} trsm_block={tb}")
     t_gen = time.perf_counter()

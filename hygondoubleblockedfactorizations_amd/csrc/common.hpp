@@ -1,6 +1,8 @@
 // Internal definitions shared by the kernels and the host drivers.
 #pragma once
 
+#include <utility>
+
 #include <hip/hip_runtime.h>
 
 #include <cstddef>
@@ -86,6 +88,17 @@ struct mifact_handle_s {
     void* pack_ws = nullptr;
     size_t pack_ws_bytes = 0;
 
+    // Second stream of the handle's device (default priority, created with
+    // the handle) for updates that overlap the panel work of the main stream
+    // inside one driver call; fork/join events; the pack arena its
+    // low-precision updates use.  Null when its creation failed: the drivers
+    // then run the same launches, in the same order, on the main stream.
+    hipStream_t side_stream = nullptr;
+    hipEvent_t side_fork = nullptr;
+    hipEvent_t side_join = nullptr;
+    void* side_pack_ws = nullptr;
+    size_t side_pack_ws_bytes = 0;
+
     // Batched path: work-queue order, counter and FP32 staging; the private
     // stream refinement graphs are captured on.
     void* batch_ws = nullptr;
@@ -129,6 +142,31 @@ void report_status(mifact_handle h, mifact_status s, const char* what);
 // Checks the launch status of the last kernel.
 #define MIFACT_LAUNCHED(h) MIFACT_HIP((h), hipGetLastError())
 
+// Launches made while an instance lives go to the handle's side stream and
+// use its side pack arena (no-op when the handle has no side stream).
+class OnSideStream {
+public:
+    explicit OnSideStream(mifact_handle h) : h_(h->side_stream != nullptr ? h : nullptr)
+    {
+        if (h_ != nullptr) swap_();
+    }
+    ~OnSideStream()
+    {
+        if (h_ != nullptr) swap_();
+    }
+    OnSideStream(const OnSideStream&) = delete;
+    OnSideStream& operator=(const OnSideStream&) = delete;
+
+private:
+    void swap_()
+    {
+        std::swap(h_->stream, h_->side_stream);
+        std::swap(h_->pack_ws, h_->side_pack_ws);
+        std::swap(h_->pack_ws_bytes, h_->side_pack_ws_bytes);
+    }
+    mifact_handle h_;
+};
+
 // ---------------------------------------------------------------- indexing
 
 // 64-bit column-major element offset: n = 65536 gives n^2 = 2^32 elements.
@@ -147,6 +185,16 @@ __host__ __device__ inline T* at(T* A, long long ld, long long row, long long co
 __host__ __device__ inline int imin(int a, int b) { return a < b ? a : b; }
 __host__ __device__ inline int imax(int a, int b) { return a > b ? a : b; }
 __host__ __device__ inline int ceil_div(int a, int b) { return (a + b - 1) / b; }
+
+// XCD of the executing workgroup (hardware register XCC_ID, bits 3:0); the
+// hardware deals consecutive workgroups of a launch to the XCDs in turn.
+constexpr int kNumXcd = 8;
+__device__ inline unsigned int xcd_id()
+{
+    unsigned int v;
+    asm volatile("s_getreg_b32 %0, hwreg(HW_REG_XCC_ID)" : "=s"(v));
+    return v & 0xfu;
+}
 
 // ---------------------------------------------------------------- workspace
 

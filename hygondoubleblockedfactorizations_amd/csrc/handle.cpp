@@ -231,6 +231,18 @@ mifact_status mifact_create_handle(mifact_handle* handle)
         delete h;
         return mifact_status_hip_error;
     }
+    // the side stream (default priority) and its fork/join events; without
+    // them the drivers keep every launch on the main stream
+    if (hipStreamCreateWithFlags(&h->side_stream, hipStreamNonBlocking) != hipSuccess) {
+        h->side_stream = nullptr;
+    } else if (hipEventCreateWithFlags(&h->side_fork, hipEventDisableTiming) != hipSuccess
+               || hipEventCreateWithFlags(&h->side_join, hipEventDisableTiming) != hipSuccess) {
+        if (h->side_fork != nullptr) (void)hipEventDestroy(h->side_fork);
+        (void)hipStreamDestroy(h->side_stream);
+        h->side_stream = nullptr;
+        h->side_fork = nullptr;
+        h->side_join = nullptr;
+    }
     *handle = h;
     return mifact_status_success;
 }
@@ -261,6 +273,18 @@ mifact_status mifact_destroy_handle(mifact_handle h)
     }
     if (h->capture_stream != nullptr) {
         (void)hipStreamDestroy(h->capture_stream);
+    }
+    if (h->side_pack_ws != nullptr) {
+        (void)hipFree(h->side_pack_ws);
+    }
+    if (h->side_fork != nullptr) {
+        (void)hipEventDestroy(h->side_fork);
+    }
+    if (h->side_join != nullptr) {
+        (void)hipEventDestroy(h->side_join);
+    }
+    if (h->side_stream != nullptr) {
+        (void)hipStreamDestroy(h->side_stream);
     }
     for (hipEvent_t e : h->event_pool) {
         (void)hipEventDestroy(e);

@@ -221,14 +221,6 @@ __device__ inline void xstore64_xcd(unsigned long long* p, unsigned int payload,
     __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
 }
 
-// XCD of the executing workgroup (hardware register XCC_ID, bits 3:0).
-__device__ inline unsigned int xcd_id()
-{
-    unsigned int v;
-    asm volatile("s_getreg_b32 %0, hwreg(HW_REG_XCC_ID)" : "=s"(v));
-    return v & 0xfu;
-}
-
 __device__ inline unsigned long long xload64(const unsigned long long* p)
 {
     return __hip_atomic_load(const_cast<unsigned long long*>(p), __ATOMIC_RELAXED,

@@ -94,6 +94,13 @@ void double_blocked_potrf_impl(mifact_handle h, T* A, int lda_i, int n, int nb, 
     }
     if (info != nullptr) clear_int(h, info);
 
+    // Outer updates are applied right-looking, block by block, as soon as an
+    // outer block is factored: block k + 1 -= L_k L_k^T on the main stream
+    // (its panel work needs it next), blocks k + 2 .. -= L_k L_k^T on the side
+    // stream, where they overlap block k + 1's panel work.  Every block
+    // receives the contributions of the blocks before it in block order, so
+    // the result does not depend on the overlap.
+    bool side_pending = false;   // a side update (it wrote block k + 1) may be running
     for (int outer = 0; outer < n; outer += nb) {
         const int outer_size = imin(nb, n - outer);
         const int outer_end = outer + outer_size;
@@ -126,17 +133,38 @@ void double_blocked_potrf_impl(mifact_handle h, T* A, int lda_i, int n, int nb, 
                              lda, at(A, lda, panel_start, panel_start), lda);
             }
         }
-        // next outer block -= L * L^T over all previous columns (:251-292):
-        // SYRK of the diagonal block + GEMM below it, fused in one
-        // lower-trapezoid launch.
+        // next outer block -= L * L^T over all previous columns (:251-292),
+        // here as the contribution of this block (SYRK of the diagonal block +
+        // GEMM below it, fused in one lower-trapezoid launch); the earlier
+        // blocks' contributions came through the side stream.
         const int next_outer = outer_end;
         if (next_outer < n) {
             const int next_size = imin(nb, n - next_outer);
-            update_nt<T>(h, prec, kShapeLower, n - next_outer, next_size, next_outer,
-                         at(A, lda, next_outer, 0), lda, at(A, lda, next_outer, 0), lda,
+            const int after = next_outer + next_size;   // first column after the next block
+            if (side_pending) MIFACT_HIP(h, hipStreamWaitEvent(h->stream, h->side_join, 0));
+            side_pending = false;
+            update_nt<T>(h, prec, kShapeLower, n - next_outer, next_size, outer_size,
+                         at(A, lda, next_outer, outer), lda, at(A, lda, next_outer, outer), lda,
                          at(A, lda, next_outer, next_outer), lda);
+            // forked after the launch above, so that the side update runs
+            // beside block k + 1's panel work rather than beside that update
+            if (after < n) {
+                if (h->side_stream != nullptr)
+                    MIFACT_HIP(h, hipEventRecord(h->side_fork, h->stream));
+                OnSideStream side(h);
+                if (h->side_join != nullptr)
+                    MIFACT_HIP(h, hipStreamWaitEvent(h->stream, h->side_fork, 0));
+                update_nt<T>(h, prec, kShapeLower, n - after, n - after, outer_size,
+                             at(A, lda, after, outer), lda, at(A, lda, after, outer), lda,
+                             at(A, lda, after, after), lda);
+                if (h->side_join != nullptr) {
+                    MIFACT_HIP(h, hipEventRecord(h->side_join, h->stream));
+                    side_pending = true;
+                }
+            }
         }
     }
+    if (side_pending) MIFACT_HIP(h, hipStreamWaitEvent(h->stream, h->side_join, 0));
 }
 
 template void double_blocked_potrf_impl<float>(mifact_handle, float*, int, int, int, int, int*,
