@@ -702,13 +702,15 @@ static void launch_leaf(mifact_handle h, LeafArgs<T>& args)
 template <typename T, int RPT>
 static void launch_leaf_np(mifact_handle h, LeafArgs<T>& args)
 {
-    // words gathered per thread: up to 16, 32 or kMaxG records
+    // words gathered per thread: up to 16, 32, 64 or kMaxG records
     constexpr int kSmall = (16 * Rec<T>::RW + kLeafThreads - 1) / kLeafThreads;
     constexpr int kMid = (32 * Rec<T>::RW + kLeafThreads - 1) / kLeafThreads;
+    constexpr int kWide = (64 * Rec<T>::RW + kLeafThreads - 1) / kLeafThreads;
     constexpr int kLarge = (kMaxG * Rec<T>::RW + kLeafThreads - 1) / kLeafThreads;
     static_assert(kLarge <= 32, "poll mask is 32 bits");
     if (args.G <= 16) launch_leaf<T, RPT, kSmall>(h, args);
     else if (args.G <= 32) launch_leaf<T, RPT, kMid>(h, args);
+    else if (args.G <= 64) launch_leaf<T, RPT, kWide>(h, args);
     else launch_leaf<T, RPT, kLarge>(h, args);
 }
 
