@@ -607,6 +607,12 @@ def batched_section(torch, lib, h, P, batch_spd, batch_lu, seed):
         return float(min(ts))
 
     out = {}
+    # size classes of the trailing updates from the autotuner (defaults when untuned)
+    from hygondoubleblockedfactorizations_amd import tuning
+    sc = tuning.lookup_batched()
+    lib.mifact_set_batched_tiles(h.ptr, sc["potrf_small_max"], sc["getrf_small_max"])
+    out["size_classes"] = dict(sc, source="tuning_batched.json" if os.path.exists(
+        tuning.BATCHED_TABLE) else "library defaults")
     for kind, batch, lo, hi in (("potrf", batch_spd, 32, 512), ("getrf", batch_lu, 64, 1024)):
         if batch <= 0:
             continue

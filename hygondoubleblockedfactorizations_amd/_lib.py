@@ -27,6 +27,7 @@ SIGNATURES = {
     "mifact_reserve_workspace": [P, I],
     "mifact_set_gemm_config": [P, I],
     "mifact_set_gemm_policy": [P, I],
+    "mifact_set_batched_tiles": [P, I, I],
     "mifact_set_wait_limit": [P, ctypes.c_longlong],
     "mifact_version": [],
     "mifact_profile_begin": [P],
@@ -120,6 +121,7 @@ _RESTYPE = {
     "mifact_reserve_workspace": I,
     "mifact_set_gemm_config": I,
     "mifact_set_gemm_policy": I,
+    "mifact_set_batched_tiles": I,
     "mifact_set_wait_limit": I,
     "mifact_version": ctypes.c_char_p,
     "mifact_profile_begin": I,

@@ -38,6 +38,7 @@ typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef _Float16 halfx8 __attribute__((ext_vector_type(8)));
 
 struct BatchArgs {
+    int small_max;       // trailing orders up to this take the 32 x 32 per-wave tiles
     float* A;
     const long long* offsets;
     const int* n;
@@ -124,13 +125,105 @@ __device__ void tile_update(float* C, long long ldc, int m, int ncols, int r0, i
     __syncthreads();
 }
 
+// Size class of the small trailing updates: one 32 x 32 tile per wave, four
+// tiles per pass, no workgroup barrier inside a pass.  Each wave stages its
+// operands (32 x kb of each, k-major) in its own 2 x 32 x 33 slice of
+// `scratch` (4 x 2112 floats).  Tiles t = 0 .. count - 1 in row-major order
+// over the lower triangle (LOWER) or the full square of 32-blocks.
+constexpr int kSmallT = 32;
+constexpr int kSmallScratch = 4 * 2 * kNB * (kSmallT + 1);
+
+template <int PREC, bool LOWER, typename LA, typename RB>
+__device__ void tile_update_small(float* C, long long ldc, int m, int kb, LA la, RB rb,
+                                  float* scratch)
+{
+    const int tid = threadIdx.x;
+    const int lane = tid & 63, wave = tid >> 6;
+    float (*sa)[kSmallT + 1] = reinterpret_cast<float (*)[kSmallT + 1]>(scratch + wave * 2 * kNB * (kSmallT + 1));
+    float (*sb)[kSmallT + 1] = sa + kNB;
+    const int tb = (m + kSmallT - 1) / kSmallT;
+    const int count = LOWER ? tb * (tb + 1) / 2 : tb * tb;
+    for (int t = wave; t < count; t += 4) {
+        int ti, tj;
+        if (LOWER) {          // t = ti (ti + 1) / 2 + tj, tj <= ti
+            ti = 0;
+            while ((ti + 1) * (ti + 2) / 2 <= t) ++ti;
+            tj = t - ti * (ti + 1) / 2;
+        } else {
+            ti = t % tb;
+            tj = t / tb;
+        }
+        const int r0 = ti * kSmallT, c0 = tj * kSmallT;
+        for (int e = lane; e < kNB * kSmallT; e += 64) {
+            const int i = e % kSmallT, k = e / kSmallT;
+            float va = 0.0f, vb = 0.0f;
+            if (k < kb) {
+                if (r0 + i < m) va = la(r0 + i, k);
+                if (c0 + i < m) vb = rb(k, c0 + i);
+            }
+            sa[k][i] = va;
+            sb[k][i] = vb;
+        }
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        floatx16 acc;
+#pragma unroll
+        for (int r = 0; r < 16; ++r) acc[r] = 0.0f;
+        const int rc = lane & 31;
+        if (PREC == kPrecFP32) {
+#pragma unroll
+            for (int kk = 0; kk < kNB; kk += 2)
+                acc = __builtin_amdgcn_mfma_f32_32x32x2f32(sa[kk + (lane >> 5)][rc],
+                                                            sb[kk + (lane >> 5)][rc], acc, 0, 0, 0);
+        } else {
+#pragma unroll
+            for (int kk = 0; kk < kNB; kk += 16) {
+                const int k0 = kk + 8 * (lane >> 5);
+                if (PREC == kPrecBF16) {
+                    bf16x8 a, b;
+#pragma unroll
+                    for (int e = 0; e < 8; ++e) {
+                        a[e] = static_cast<__bf16>(sa[k0 + e][rc]);
+                        b[e] = static_cast<__bf16>(sb[k0 + e][rc]);
+                    }
+                    acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, b, acc, 0, 0, 0);
+                } else {
+                    halfx8 a, b;
+#pragma unroll
+                    for (int e = 0; e < 8; ++e) {
+                        a[e] = static_cast<_Float16>(sa[k0 + e][rc]);
+                        b[e] = static_cast<_Float16>(sb[k0 + e][rc]);
+                    }
+                    acc = __builtin_amdgcn_mfma_f32_32x32x16_f16(a, b, acc, 0, 0, 0);
+                }
+            }
+        }
+        // accumulator r: row 8 (r / 4) + 4 (lane / 32) + r % 4, column lane % 32
+        const int gc = c0 + rc;
+        if (gc < m) {
+#pragma unroll
+            for (int r = 0; r < 16; ++r) {
+                const int gr = r0 + 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3);
+                if (gr < m && (!LOWER || gr >= gc)) C[off(gr, gc, ldc)] -= acc[r];
+            }
+        }
+        // the wave's slice is restaged by its next tile
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    }
+    __syncthreads();
+}
+
 // ---------------------------------------------------------------- Cholesky
 
 template <int PREC>
-__device__ void chol_matrix(float* A, long long lda, int n, int* info)
+__device__ void chol_matrix(float* A, long long lda, int n, int* info, int small_max)
 {
     __shared__ float d[kNB][kNB + 1];
-    __shared__ float xs[256][kNB + 1];
+    __shared__ float xs[256][kNB + 1];   // TRSM row slots; the small tiles' staging
+    static_assert(256 * (kNB + 1) >= kSmallScratch, "small-tile staging inside the row slots");
     __shared__ TileShared ts;
     __shared__ int failed;
     const int tid = threadIdx.x;
@@ -200,6 +293,10 @@ __device__ void chol_matrix(float* A, long long lda, int n, int* info)
         const float* Pp = A + off(r1, k, lda);
         auto la = [&](int i, int kk) { return Pp[off(i, kk, lda)]; };
         auto rb = [&](int kk, int j) { return Pp[off(j, kk, lda)]; };
+        if (m <= small_max) {
+            tile_update_small<PREC, true>(C, lda, m, kb, la, rb, &xs[0][0]);
+            continue;
+        }
         const int tiles = (m + kT - 1) / kT;
         for (int tj = 0; tj < tiles; ++tj)
             for (int ti = tj; ti < tiles; ++ti)
@@ -228,8 +325,9 @@ constexpr int kPanelLd = kNB + 1;   // LDS row stride: conflict-free column walk
 // Every global access is to (row, column) of the n x n matrix with both
 // below n (the caller checks lda >= n), every LDS access below its bounds.
 template <int PREC>
-__device__ void lu_matrix(float* A, long long lda, int n, int* ipiv, int* info)
+__device__ void lu_matrix(float* A, long long lda, int n, int* ipiv, int* info, int small_max)
 {
+    static_assert(kLuMaxOrder * kPanelLd >= kSmallScratch, "small-tile staging inside the panel");
     __shared__ float pnl[kLuMaxOrder][kPanelLd];
     __shared__ float wv[4];
     __shared__ int wi[4];
@@ -349,6 +447,11 @@ __device__ void lu_matrix(float* A, long long lda, int n, int* ipiv, int* info)
         const float* U12 = A + off(k, c1, lda);
         auto la = [&](int i, int kk) { return L21[off(i, kk, lda)]; };
         auto rb = [&](int kk, int jc) { return U12[off(kk, jc, lda)]; };
+        if (mr <= small_max) {
+            // the panel array is free once U12 is solved
+            tile_update_small<PREC, false>(C, lda, mr, kb, la, rb, &pnl[0][0]);
+            continue;
+        }
         const int tiles = (mr + kT - 1) / kT;
         for (int tj = 0; tj < tiles; ++tj)
             for (int ti = 0; ti < tiles; ++ti)
@@ -386,8 +489,8 @@ __global__ __launch_bounds__(256) void batched_kernel(BatchArgs a)
         float* A = a.A + a.offsets[id];
         if (threadIdx.x == 0) *info = 0;
         __syncthreads();
-        if constexpr (LU) lu_matrix<PREC>(A, lda, n, a.ipiv + a.ipiv_offsets[id], info);
-        else chol_matrix<PREC>(A, lda, n, info);
+        if constexpr (LU) lu_matrix<PREC>(A, lda, n, a.ipiv + a.ipiv_offsets[id], info, a.small_max);
+        else chol_matrix<PREC>(A, lda, n, info, a.small_max);
     }
 }
 
@@ -657,7 +760,8 @@ void launch_batched(mifact_handle h, bool lu, LowPrecision prec, int batch, floa
     hipLaunchKernelGGL(order_kernel, dim3(1), dim3(1024), 0, h->stream, n, batch, s.order,
                        s.counter);
     MIFACT_LAUNCHED(h);
-    BatchArgs a{A, offsets, n, lda, ipiv, ipiv_offsets, info, s.order, s.counter, batch};
+    BatchArgs a{lu ? h->batched_small_max_lu : h->batched_small_max_chol,
+                A, offsets, n, lda, ipiv, ipiv_offsets, info, s.order, s.counter, batch};
     // the LU workgroup holds a 1024 x 32 panel in LDS: one per CU
     const int grid = imax(1, imin(batch, h->num_cu * (lu ? 1 : 4)));
     StageScope scope(h, kStageBatched, 0.0, 0.0);

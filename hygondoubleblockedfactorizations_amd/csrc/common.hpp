@@ -104,6 +104,11 @@ struct mifact_handle_s {
     void* batch_ws = nullptr;
     size_t batch_ws_bytes = 0;
     hipStream_t capture_stream = nullptr;
+    // Size classes of the batched kernels' trailing updates: orders up to
+    // these take 32 x 32 tiles (one per wave), larger ones 64 x 64 tiles on
+    // four waves (see mifact_set_batched_tiles).
+    int batched_small_max_chol = 96;
+    int batched_small_max_lu = 96;
 
     // Stage profiler: event pool and the records of the current session.
     bool profiling = false;

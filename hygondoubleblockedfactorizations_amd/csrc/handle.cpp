@@ -371,6 +371,15 @@ mifact_status mifact_set_gemm_policy(mifact_handle h, int policy)
     return mifact_status_success;
 }
 
+mifact_status mifact_set_batched_tiles(mifact_handle h, int potrf_small_max, int getrf_small_max)
+{
+    if (h == nullptr) return mifact_status_invalid_handle;
+    if (potrf_small_max < 0 || getrf_small_max < 0) return mifact_status_invalid_value;
+    h->batched_small_max_chol = potrf_small_max;
+    h->batched_small_max_lu = getrf_small_max;
+    return mifact_status_success;
+}
+
 mifact_status mifact_set_wait_limit(mifact_handle h, long long polls)
 {
     if (h == nullptr) return mifact_status_invalid_handle;

@@ -12,7 +12,15 @@ fastest is stored in ``tuning_table.json`` next to this file, keyed by n.
 ``lookup(n)`` returns the entry of the nearest tuned order (block sizes
 clipped to n), or the built-in defaults when no table exists.
 
+The batched factorizations' size classes (mifact_set_batched_tiles: the
+largest trailing order updated in 32 x 32 per-wave tiles rather than 64 x 64
+tiles on four waves) are tuned the same way on the batched workloads --
+4096 SPD matrices with n ~ U[32, 512] and 2048 general matrices with
+n ~ U[64, 1024] -- and stored in ``tuning_batched.json``
+(``lookup_batched()``).
+
     python -m hygondoubleblockedfactorizations_amd.tuning --n 8192 16384 32768
+    python -m hygondoubleblockedfactorizations_amd.tuning --batched
 """
 from __future__ import annotations
 
@@ -23,6 +31,7 @@ import sys
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 TABLE = os.path.join(HERE, "tuning_table.json")
+BATCHED_TABLE = os.path.join(HERE, "tuning_batched.json")
 
 # Candidate grids (outer, inner) per routine; the outer-TRSM block of GETRF.
 POTRF_GRID = [(2048, 256), (4096, 256), (4096, 512), (8192, 256), (8192, 512), (8192, 1024),
@@ -31,6 +40,11 @@ GETRF_GRID = [(4096, 2048), (4096, 4096), (6144, 3072), (8192, 2048), (8192, 409
               (16384, 4096)]
 TRSM_BLOCKS = [2048, 4096]
 GEMM_POLICIES = [0, 1, 2, 3]   # mifact_set_gemm_policy
+# largest trailing order of the small size class (mifact_set_batched_tiles)
+BATCHED_SMALL = [0, 64, 96, 128, 192, 256, 384]
+BATCHED_DEFAULT = {"potrf_small_max": 96, "getrf_small_max": 96}
+# (routine, batch, smallest order, largest order) of the batched workloads
+BATCHED_WORKLOADS = [("potrf", 4096, 32, 512), ("getrf", 2048, 64, 1024)]
 
 
 def defaults(n: int) -> dict:
@@ -155,12 +169,98 @@ def tune(n: int, reps: int = 2, log=print) -> dict:
     return best
 
 
+def lookup_batched(path: str = BATCHED_TABLE) -> dict:
+    """Tuned size classes of the batched factorizations (the library's
+    defaults when nothing is tuned)."""
+    if not os.path.exists(path):
+        return dict(BATCHED_DEFAULT)
+    with open(path) as f:
+        e = json.load(f)
+    return {k: int(e.get(k, v)) for k, v in BATCHED_DEFAULT.items()}
+
+
+def batched_inputs(torch, routine, batch, lo, hi, seed):
+    """Packed column-major batch (lda = n) of the batched workloads: SPD
+    A = G G^T / n + I, or iid U(-1, 1); returns (flat, offsets, n, ipiv
+    offsets) as device tensors and the flop count."""
+    import numpy as np
+    rng = np.random.Generator(np.random.PCG64(seed))
+    ns = rng.integers(lo, hi + 1, size=batch).astype(np.int32)
+    sizes = ns.astype(np.int64) ** 2
+    offs = np.concatenate([[0], np.cumsum(sizes)[:-1]]).astype(np.int64)
+    g = torch.Generator(device="cuda")
+    g.manual_seed(seed)
+    if routine == "potrf":
+        flat = torch.empty(int(sizes.sum()), dtype=torch.float32, device="cuda")
+        for i, n in enumerate(ns):
+            n = int(n)
+            G = torch.randn(n, n, device="cuda", generator=g, dtype=torch.float64)
+            a = (G @ G.t() / n + torch.eye(n, device="cuda", dtype=torch.float64)).float()
+            flat[offs[i]:offs[i] + n * n] = a.reshape(-1)
+        flops = float((ns.astype(np.float64) ** 3).sum() / 3.0)
+    else:
+        flat = torch.rand(int(sizes.sum()), dtype=torch.float32, device="cuda",
+                          generator=g).mul_(2).sub_(1)
+        flops = float((2.0 * ns.astype(np.float64) ** 3).sum() / 3.0)
+    poff = np.concatenate([[0], np.cumsum(ns)[:-1]]).astype(np.int64)
+    return (flat, torch.from_numpy(offs).cuda(), torch.from_numpy(ns).cuda(),
+            torch.from_numpy(poff).cuda(), flops)
+
+
+def tune_batched(reps: int = 3, log=print) -> dict:
+    """Times every size-class threshold on the batched workloads."""
+    import torch
+    from . import _lib
+    lib = _lib.load()
+    h = _lib.Handle(stream=torch.cuda.current_stream())
+    P = _lib.ptr
+    best = {}
+    for routine, batch, lo, hi in BATCHED_WORKLOADS:
+        flat, d_off, d_n, d_poff, flops = batched_inputs(torch, routine, batch, lo, hi,
+                                                         20260716 + (0 if routine == "potrf" else 1))
+        work = torch.empty_like(flat)
+        info = torch.zeros(batch, dtype=torch.int32, device="cuda")
+        ipiv = torch.zeros(int(d_n.sum().item()), dtype=torch.int32, device="cuda")
+        res = []
+        for small in BATCHED_SMALL:
+            other = BATCHED_DEFAULT["getrf_small_max" if routine == "potrf" else "potrf_small_max"]
+            if routine == "potrf":
+                lib.mifact_set_batched_tiles(h.ptr, small, other)
+                fn = lambda: lib.mifact_potrf_batched(h.ptr, batch, P(work), P(d_off), P(d_n),
+                                                      P(d_n), P(info))
+            else:
+                lib.mifact_set_batched_tiles(h.ptr, other, small)
+                fn = lambda: lib.mifact_getrf_batched(h.ptr, batch, P(work), P(d_off), P(d_n),
+                                                      P(d_n), P(ipiv), P(d_poff), P(info))
+            ms = _time(torch, fn, lambda: work.copy_(flat), reps)
+            res.append((ms, small))
+            log(f"  batched {routine} small_max={small}: {ms:.3f} ms "
+                f"({flops / ms / 1e9:.2f} TFLOP/s)")
+        ms, small = min(res)
+        best[f"{routine}_small_max"] = small
+        best[f"{routine}_ms"] = ms
+        best[f"{routine}_candidates_ms"] = {str(s): m for m, s in res}
+    if h.last_error() != 0:
+        raise RuntimeError("library reported an error while tuning")
+    best["workload"] = ("4096 SPD n~U[32,512] (A = G G^T/n + I); 2048 general n~U[64,1024] "
+                        "(iid U(-1,1)); FP32, lda = n")
+    return best
+
+
 def main(argv=None):
     ap = argparse.ArgumentParser()
     ap.add_argument("--n", type=int, nargs="+", default=[8192, 16384, 32768])
     ap.add_argument("--reps", type=int, default=2)
     ap.add_argument("--out", default=TABLE)
+    ap.add_argument("--batched", action="store_true",
+                    help="tune the batched size classes instead (tuning_batched.json)")
     args = ap.parse_args(argv)
+    if args.batched:
+        best = tune_batched(max(args.reps, 3), log=lambda s: print(s, flush=True))
+        print(f"  best: {best}", flush=True)
+        with open(BATCHED_TABLE, "w") as f:
+            json.dump(best, f, indent=1)
+        return 0
     table = load_table(args.out)
     for n in args.n:
         print(f"tuning n={n}", flush=True)

@@ -83,6 +83,14 @@ mifact_status mifact_set_gemm_config(mifact_handle handle, int config);
  * updates; 3 = as 0 but 128 x 128 from 1 tile per CU. */
 mifact_status mifact_set_gemm_policy(mifact_handle handle, int policy);
 
+/* Size classes of the batched factorizations' trailing updates (an
+ * autotuner knob): a trailing matrix of order up to `potrf_small_max`
+ * (Cholesky) or `getrf_small_max` (LU) is updated in 32 x 32 tiles, one per
+ * wave; larger ones in 64 x 64 tiles on the workgroup's four waves.
+ * Defaults 96 / 96; 0 keeps every update on the 64 x 64 tiles. */
+mifact_status mifact_set_batched_tiles(mifact_handle handle, int potrf_small_max,
+                                       int getrf_small_max);
+
 /* Library version string. */
 const char* mifact_version(void);
 

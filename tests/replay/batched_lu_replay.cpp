@@ -1,7 +1,8 @@
 // Host replay of the batched LU kernel's address arithmetic
 // (hygondoubleblockedfactorizations_amd/csrc/batched.hip: order_kernel,
 // batched_kernel<kPrecFP32, true>, lu_matrix with its panel in LDS,
-// tile_update<kPrecFP32, false>).
+// tile_update<kPrecFP32, false> and, for trailing orders up to the small
+// size class, tile_update_small<kPrecFP32, false> staging in the panel).
 //
 // Every phase between two workgroup barriers runs for tid = 0..255 in turn
 // (or, where the phase's accesses do not depend on the thread split, as one
@@ -16,19 +17,22 @@
 // Input (little endian): int32 batch, int64 total, int32 n[batch],
 // int32 lda[batch], int64 offsets[batch], int64 ipiv_offsets[batch],
 // float flat[total].  Output: float flat[total], int32 ipiv[sum max(n,1)],
-// int32 info[batch].  Prints the count of out-of-range accesses; exit 1 if
-// any.
+// int32 info[batch].  Optional third argument: the small size class's
+// largest trailing order (the library's default, 96, when absent).  Prints
+// the count of out-of-range accesses; exit 1 if any.
 #include <algorithm>
 #include <cfloat>
 #include <cmath>
 #include <cstdint>
 #include <cstdio>
+#include <cstdlib>
 #include <vector>
 
 namespace {
 
 constexpr int kNB = 32, kT = 64, kMaxOrder = 4096, kLuMaxOrder = 1024, kBuckets = 256,
-              kThreads = 256;
+              kThreads = 256, kSmallT = 32;
+int small_max = 96;
 
 struct Checked {
     std::vector<float> flat;
@@ -113,9 +117,50 @@ void tile_update(Checked& M, int c1, int k, int m, int r0, int c0, int kb)
     }
 }
 
+using Panel = Lds<float, kLuMaxOrder, kNB + 1>;
+
+// The small size class: wave w takes tiles w, w + 4, ... of the 32-blocks of
+// the trailing matrix (column-major tile order), stages its operands in its
+// 2 x 32 x 33 slice of the panel array and writes its 32 x 32 tile.
+void tile_update_small(Checked& M, Panel& pnl, int c1, int k, int m, int kb)
+{
+    const int tb = (m + kSmallT - 1) / kSmallT, count = tb * tb;
+    auto slot = [&](int wave, int half, int kk, int i) -> float& {
+        const int idx = wave * 2 * kNB * (kSmallT + 1) + half * kNB * (kSmallT + 1)
+                        + kk * (kSmallT + 1) + i;
+        return pnl.at(idx / (kNB + 1), idx % (kNB + 1));
+    };
+    for (int wave = 0; wave < 4; ++wave)
+        for (int t = wave; t < count; t += 4) {
+            const int r0 = (t % tb) * kSmallT, c0 = (t / tb) * kSmallT;
+            for (int lane = 0; lane < 64; ++lane)
+                for (int e = lane; e < kNB * kSmallT; e += 64) {
+                    const int i = e % kSmallT, kk = e / kSmallT;
+                    float va = 0.0f, vb = 0.0f;
+                    if (kk < kb) {
+                        if (r0 + i < m) va = M.A(c1 + r0 + i, k + kk);
+                        if (c0 + i < m) vb = M.A(k + kk, c1 + c0 + i);
+                    }
+                    slot(wave, 0, kk, i) = va;
+                    slot(wave, 1, kk, i) = vb;
+                }
+            for (int lane = 0; lane < 64; ++lane) {
+                const int col = lane & 31, gc = c0 + col;
+                if (gc >= m) continue;
+                for (int r = 0; r < 16; ++r) {
+                    const int lr = 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3);
+                    float acc = 0.0f;
+                    for (int kk = 0; kk < kNB; ++kk)
+                        acc = std::fma(slot(wave, 0, kk, lr), slot(wave, 1, kk, col), acc);
+                    if (r0 + lr < m) M.A(c1 + r0 + lr, c1 + gc) -= acc;
+                }
+            }
+        }
+}
+
 void lu_matrix(Checked& M, int n, int* info)
 {
-    static Lds<float, kLuMaxOrder, kNB + 1> pnl;
+    static Panel pnl;
     Lds<int, 1, kNB> piv_s;
     pnl.violations = piv_s.violations = &M.violations;
     for (int k = 0; k < n; k += kNB) {
@@ -220,6 +265,10 @@ void lu_matrix(Checked& M, int n, int* info)
                 for (int ii = 0; ii < kb; ++ii) M.A(k + ii, c) = pnl.at(kNB + tid, ii);
             }
         // 4. A22 -= L21 U12
+        if (mr <= small_max) {
+            tile_update_small(M, pnl, c1, k, mr, kb);
+            continue;
+        }
         const int tiles = (mr + kT - 1) / kT;
         for (int tj = 0; tj < tiles; ++tj)
             for (int ti = 0; ti < tiles; ++ti) tile_update(M, c1, k, mr, ti * kT, tj * kT, kb);
@@ -236,10 +285,11 @@ bool get(FILE* f, T* p, size_t count)
 
 int main(int argc, char** argv)
 {
-    if (argc != 3) {
-        std::fprintf(stderr, "usage: %s input output\n", argv[0]);
+    if (argc != 3 && argc != 4) {
+        std::fprintf(stderr, "usage: %s input output [small_max]\n", argv[0]);
         return 2;
     }
+    if (argc == 4) small_max = std::atoi(argv[3]);
     FILE* f = std::fopen(argv[1], "rb");
     if (f == nullptr) return 2;
     int batch = 0;

@@ -53,15 +53,19 @@ def packed_case(lo, hi, batch):
     return mats, n, lda, off, piv_off, flat
 
 
+@pytest.mark.parametrize("small_max", [0, 96, 1024])
 @pytest.mark.parametrize("lo,hi,batch", [(1, 96, 40), (64, 512, 16), (700, 1024, 3)])
-def test_batched_lu_addresses_in_range(replay, tmp_path, lo, hi, batch):
+def test_batched_lu_addresses_in_range(replay, tmp_path, lo, hi, batch, small_max):
+    """small_max: largest trailing order updated by the small size class's
+    32 x 32 per-wave tiles (0: none, 96: the default, 1024: every update)."""
     mats, n, lda, off, piv_off, flat = packed_case(lo, hi, batch)
     inp, out = tmp_path / "in.bin", tmp_path / "out.bin"
     with open(inp, "wb") as f:
         f.write(struct.pack("<iq", batch, flat.size))
         for a in (n, lda, off, piv_off, flat):
             f.write(a.tobytes())
-    r = subprocess.run([replay, str(inp), str(out)], capture_output=True, text=True)
+    r = subprocess.run([replay, str(inp), str(out), str(small_max)], capture_output=True,
+                       text=True)
     assert r.returncode == 0, r.stdout + r.stderr
     assert "out-of-range accesses 0" in r.stdout
     raw = open(out, "rb").read()

@@ -78,9 +78,27 @@ def random_sizes(batch, lo, hi, seed):
     return O.rng(seed).integers(lo, hi + 1, size=batch)
 
 
+@pytest.fixture
+def tiles(H):
+    """Sets the batched size classes for one test (None: the defaults) and
+    restores the defaults afterwards."""
+    def set_(small):
+        if small is not None:
+            assert lib().mifact_set_batched_tiles(H.ptr, small, small) == 0
+    yield set_
+    assert lib().mifact_set_batched_tiles(H.ptr, 96, 96) == 0
+
+
+# small = largest trailing order on the 32 x 32 per-wave tiles: 0 keeps every
+# update on the 64 x 64 tiles, 4096 puts every update on the small tiles
+SIZE_CLASSES = [None, 0, 4096]
+
+
+@pytest.mark.parametrize("small", SIZE_CLASSES)
 @pytest.mark.parametrize("lo,hi,batch", [(1, 96, 40), (32, 512, 24), (600, 1100, 3)])
-def test_potrf_batched(H, lo, hi, batch):
-    log("potrf_batched", lo, hi, batch)
+def test_potrf_batched(H, tiles, lo, hi, batch, small):
+    log("potrf_batched", lo, hi, batch, small)
+    tiles(small)
     sizes = random_sizes(batch, lo, hi, 100 + hi)
     mats = [O.spd_matrix(int(n), 1000 + i)[:n, :n] for i, n in enumerate(sizes)]
     pk = Packed(mats, [i % 3 for i in range(batch)])
@@ -131,9 +149,11 @@ def test_potrf_batched_info(H):
         assert O.potrf_residual(np.asfortranarray(F[:n]), np.asfortranarray(mats[i])) <= 1e-5
 
 
+@pytest.mark.parametrize("small", SIZE_CLASSES)
 @pytest.mark.parametrize("lo,hi,batch", [(1, 96, 40), (64, 512, 16), (700, 1024, 3)])
-def test_getrf_batched(H, lo, hi, batch):
-    log("getrf_batched", lo, hi, batch)
+def test_getrf_batched(H, tiles, lo, hi, batch, small):
+    log("getrf_batched", lo, hi, batch, small)
+    tiles(small)
     sizes = random_sizes(batch, lo, hi, 200 + hi)
     mats = [O.uniform_matrix(int(n), 3000 + i) for i, n in enumerate(sizes)]
     pk = Packed(mats, [(i * 5) % 4 for i in range(batch)])
