@@ -51,77 +51,115 @@ struct BatchArgs {
     int batch;
 };
 
-struct TileShared {
-    float a[kNB][kT + 1];    // a[k][row]: left operand (rows of the panel)
-    float b[kNB][kT + 1];    // b[k][col]: right operand
-};
+// The 64 x 64 tiles of a trailing update C -= L R, one after another on the
+// workgroup's four waves (2 x 2 blocks of v_mfma_f32_32x32x*), software
+// pipelined: while tile t's products run, the operands of tile t + 1 are
+// read from global memory into registers, and tile t's block of C is read
+// before its products; two operand buffers (k-major, 2 x kNB x 65 floats
+// each) alternate in `scratch`, one workgroup barrier per tile.
+// la(i, k): left operand element (row i); rb(k, j): right operand element
+// (column j).  LOWER: only the tiles and elements with row >= column.
+constexpr int kTileFloats = 2 * kNB * (kT + 1);
 
-// C(r0:r0+64, c0:c0+64) -= L(r0:, 0:kb) R(0:kb, c0:)   on MFMA.
-// la(i, k): left operand element (row r0 + i); rb(k, j): right operand
-// element (column c0 + j).  LOWER: only C(i, j) with global row >= column.
 template <int PREC, bool LOWER, typename LA, typename RB>
-__device__ void tile_update(float* C, long long ldc, int m, int ncols, int r0, int c0, int kb,
-                            LA la, RB rb, TileShared& s)
+__device__ void tile_sweep(float* C, long long ldc, int m, int kb, LA la, RB rb, float* scratch)
 {
     const int tid = threadIdx.x;
     const int lane = tid & 63, wave = tid >> 6;
     const int wm = wave & 1, wn = wave >> 1;
-    // stage (zero beyond the matrix and beyond kb)
-    for (int e = tid; e < kNB * kT; e += 256) {
-        const int k = e / kT, i = e % kT;
-        float va = 0.0f, vb = 0.0f;
-        if (k < kb) {
-            if (r0 + i < m) va = la(r0 + i, k);
-            if (c0 + i < ncols) vb = rb(k, c0 + i);
-        }
-        s.a[k][i] = va;
-        s.b[k][i] = vb;
-    }
-    __syncthreads();
-    floatx16 acc;
+    const int tiles = (m + kT - 1) / kT;
+    constexpr int PER = kNB * kT / 256;   // operand elements per thread and operand
+    float na[PER], nbv[PER];
+    auto fetch = [&](int r0, int c0) {
 #pragma unroll
-    for (int r = 0; r < 16; ++r) acc[r] = 0.0f;
-    const int row = wm * 32 + (lane & 31);
-    const int col = wn * 32 + (lane & 31);
-    if (PREC == kPrecFP32) {
-#pragma unroll
-        for (int kk = 0; kk < kNB; kk += 2) {
-            const float a = s.a[kk + (lane >> 5)][row];
-            const float b = s.b[kk + (lane >> 5)][col];
-            acc = __builtin_amdgcn_mfma_f32_32x32x2f32(a, b, acc, 0, 0, 0);
-        }
-    } else {
-#pragma unroll
-        for (int kk = 0; kk < kNB; kk += 16) {
-            const int k0 = kk + 8 * (lane >> 5);
-            if (PREC == kPrecBF16) {
-                bf16x8 a, b;
-#pragma unroll
-                for (int e = 0; e < 8; ++e) {
-                    a[e] = static_cast<__bf16>(s.a[k0 + e][row]);
-                    b[e] = static_cast<__bf16>(s.b[k0 + e][col]);
-                }
-                acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, b, acc, 0, 0, 0);
-            } else {
-                halfx8 a, b;
-#pragma unroll
-                for (int e = 0; e < 8; ++e) {
-                    a[e] = static_cast<_Float16>(s.a[k0 + e][row]);
-                    b[e] = static_cast<_Float16>(s.b[k0 + e][col]);
-                }
-                acc = __builtin_amdgcn_mfma_f32_32x32x16_f16(a, b, acc, 0, 0, 0);
+        for (int q = 0; q < PER; ++q) {
+            const int e = tid + 256 * q;
+            const int k = e / kT, i = e % kT;
+            float va = 0.0f, vb = 0.0f;
+            if (k < kb) {
+                if (r0 + i < m) va = la(r0 + i, k);
+                if (c0 + i < m) vb = rb(k, c0 + i);
             }
+            na[q] = va;
+            nbv[q] = vb;
         }
-    }
-    // accumulator r: row 8 (r / 4) + 4 (lane / 32) + r % 4, column lane % 32
-    const int gc = c0 + col;
-    if (gc < ncols) {
+    };
+    int ti = 0, tj = 0;
+    fetch(0, 0);
+    for (int t = 0;; ++t) {
+        float (*sa)[kT + 1] = reinterpret_cast<float (*)[kT + 1]>(scratch + (t & 1) * kTileFloats);
+        float (*sb)[kT + 1] = sa + kNB;
+#pragma unroll
+        for (int q = 0; q < PER; ++q) {
+            const int e = tid + 256 * q;
+            sa[e / kT][e % kT] = na[q];
+            sb[e / kT][e % kT] = nbv[q];
+        }
+        __syncthreads();
+        const int r0 = ti * kT, c0 = tj * kT;
+        int ni = ti + 1, nj = tj;               // next tile: down the column, then the next one
+        if (ni >= tiles) {
+            ++nj;
+            ni = LOWER ? nj : 0;
+        }
+        const bool more = nj < tiles;
+        const int row = wm * 32 + (lane & 31);
+        const int col = wn * 32 + (lane & 31);
+        // this tile's C block (accumulator r: row 8 (r / 4) + 4 (lane / 32) + r % 4),
+        // read before the next tile's operands: loads complete in issue
+        // order, so the store below waits for these only
+        const int gc = c0 + col;
+        float cv[16];
 #pragma unroll
         for (int r = 0; r < 16; ++r) {
             const int gr = r0 + wm * 32 + 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3);
-            if (gr < m && (!LOWER || gr >= gc)) C[off(gr, gc, ldc)] -= acc[r];
+            const bool ok = gc < m && gr < m && (!LOWER || gr >= gc);
+            cv[r] = ok ? C[off(gr, gc, ldc)] : 0.0f;
         }
+        if (more) fetch(ni * kT, nj * kT);
+        floatx16 acc;
+#pragma unroll
+        for (int r = 0; r < 16; ++r) acc[r] = 0.0f;
+        if (PREC == kPrecFP32) {
+#pragma unroll
+            for (int kk = 0; kk < kNB; kk += 2) {
+                const float a = sa[kk + (lane >> 5)][row];
+                const float b = sb[kk + (lane >> 5)][col];
+                acc = __builtin_amdgcn_mfma_f32_32x32x2f32(a, b, acc, 0, 0, 0);
+            }
+        } else {
+#pragma unroll
+            for (int kk = 0; kk < kNB; kk += 16) {
+                const int k0 = kk + 8 * (lane >> 5);
+                if (PREC == kPrecBF16) {
+                    bf16x8 a, b;
+#pragma unroll
+                    for (int e = 0; e < 8; ++e) {
+                        a[e] = static_cast<__bf16>(sa[k0 + e][row]);
+                        b[e] = static_cast<__bf16>(sb[k0 + e][col]);
+                    }
+                    acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, b, acc, 0, 0, 0);
+                } else {
+                    halfx8 a, b;
+#pragma unroll
+                    for (int e = 0; e < 8; ++e) {
+                        a[e] = static_cast<_Float16>(sa[k0 + e][row]);
+                        b[e] = static_cast<_Float16>(sb[k0 + e][col]);
+                    }
+                    acc = __builtin_amdgcn_mfma_f32_32x32x16_f16(a, b, acc, 0, 0, 0);
+                }
+            }
+        }
+#pragma unroll
+        for (int r = 0; r < 16; ++r) {
+            const int gr = r0 + wm * 32 + 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3);
+            if (gc < m && gr < m && (!LOWER || gr >= gc)) C[off(gr, gc, ldc)] = cv[r] - acc[r];
+        }
+        if (!more) break;
+        ti = ni;
+        tj = nj;
     }
+    // every wave is done with the buffers and with C before the caller goes on
     __syncthreads();
 }
 
@@ -222,9 +260,9 @@ template <int PREC>
 __device__ void chol_matrix(float* A, long long lda, int n, int* info, int small_max)
 {
     __shared__ float d[kNB][kNB + 1];
-    __shared__ float xs[256][kNB + 1];   // TRSM row slots; the small tiles' staging
+    __shared__ float xs[256][kNB + 1];   // TRSM row slots; the trailing update's operand buffers
     static_assert(256 * (kNB + 1) >= kSmallScratch, "small-tile staging inside the row slots");
-    __shared__ TileShared ts;
+    static_assert(256 * (kNB + 1) >= 2 * kTileFloats, "tile buffers inside the row slots");
     __shared__ int failed;
     const int tid = threadIdx.x;
     if (tid == 0) failed = 0;
@@ -297,10 +335,7 @@ __device__ void chol_matrix(float* A, long long lda, int n, int* info, int small
             tile_update_small<PREC, true>(C, lda, m, kb, la, rb, &xs[0][0]);
             continue;
         }
-        const int tiles = (m + kT - 1) / kT;
-        for (int tj = 0; tj < tiles; ++tj)
-            for (int ti = tj; ti < tiles; ++ti)
-                tile_update<PREC, true>(C, lda, m, m, ti * kT, tj * kT, kb, la, rb, ts);
+        tile_sweep<PREC, true>(C, lda, m, kb, la, rb, &xs[0][0]);
     }
 }
 
@@ -328,11 +363,11 @@ template <int PREC>
 __device__ void lu_matrix(float* A, long long lda, int n, int* ipiv, int* info, int small_max)
 {
     static_assert(kLuMaxOrder * kPanelLd >= kSmallScratch, "small-tile staging inside the panel");
+    static_assert(kLuMaxOrder * kPanelLd >= 2 * kTileFloats, "tile buffers inside the panel");
     __shared__ float pnl[kLuMaxOrder][kPanelLd];
     __shared__ float wv[4];
     __shared__ int wi[4];
     __shared__ int piv_s[kNB];
-    __shared__ TileShared ts;
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     constexpr float sfmin = FLT_MIN;
     for (int k = 0; k < n; k += kNB) {
@@ -452,10 +487,8 @@ __device__ void lu_matrix(float* A, long long lda, int n, int* ipiv, int* info, 
             tile_update_small<PREC, false>(C, lda, mr, kb, la, rb, &pnl[0][0]);
             continue;
         }
-        const int tiles = (mr + kT - 1) / kT;
-        for (int tj = 0; tj < tiles; ++tj)
-            for (int ti = 0; ti < tiles; ++ti)
-                tile_update<PREC, false>(C, lda, mr, mr, ti * kT, tj * kT, kb, la, rb, ts);
+        // the panel array is free once U12 is solved
+        tile_sweep<PREC, false>(C, lda, mr, kb, la, rb, &pnl[0][0]);
     }
 }
 

@@ -1216,20 +1216,28 @@ mifact_status refine_solve(mifact_handle h, bool sym, int n, int nrhs, const flo
         res->diagonal_shift = shift;
         if (!attempt(prec, true, shift)) return finish(h->last_error);
     }
-    if (res->info == 0 && !res->converged && method != mifact_refine_classical) {
+    // GMRES-IR from the current iterate with the current factors as the
+    // preconditioner, when the classical steps stopped short
+    auto krylov = [&]() -> bool {
+        if (res->info != 0 || res->converged || method == mifact_refine_classical) return true;
         const Precond pc{sym, n, F, ldf, &w, w.rs, cs, pinv};
         double berr = 0.0;
         // ||A||_inf for the FGMRES backward errors (bits in norms[3])
         fill<unsigned long long>(h, w.norms + 3, 1, 1, 1, 0ull);
         norm_inf(h, sym, A, lda, n, w.part, w.norms + 3);
         if (!gmres_refine(h, pc, A, lda, nrhs, B, ldb, X, ldx, tol, max_it, res, &berr))
-            return finish(h->last_error);
+            return false;
         res->backward_error = berr;
-    }
+        return true;
+    };
+    if (!krylov()) return finish(h->last_error);
     if (!res->converged && prec != kPrecFP32 && prm->fallback_fp32 != 0) {
+        // FP32 factors; at large n and kappa (n u_32 kappa > 1) their
+        // classical steps can stall too, and FGMRES finishes
         res->fell_back = 1;
         res->diagonal_shift = 0.0;
         if (!attempt(kPrecFP32, false, 0.0f)) return finish(h->last_error);
+        if (!krylov()) return finish(h->last_error);
     }
     return finish(mifact_status_success);
 }

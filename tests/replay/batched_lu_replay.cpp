@@ -1,8 +1,9 @@
 // Host replay of the batched LU kernel's address arithmetic
 // (hygondoubleblockedfactorizations_amd/csrc/batched.hip: order_kernel,
 // batched_kernel<kPrecFP32, true>, lu_matrix with its panel in LDS,
-// tile_update<kPrecFP32, false> and, for trailing orders up to the small
-// size class, tile_update_small<kPrecFP32, false> staging in the panel).
+// tile_sweep<kPrecFP32, false> with its two operand buffers in the panel
+// array and, for trailing orders up to the small size class,
+// tile_update_small<kPrecFP32, false> staging in the panel).
 //
 // Every phase between two workgroup barriers runs for tid = 0..255 in turn
 // (or, where the phase's accesses do not depend on the thread split, as one
@@ -85,39 +86,54 @@ struct Lds {
     }
 };
 
-// C(c1 + r0 .., c1 + c0 ..) -= L21 U12 on one 64 x 64 tile (the MFMA
-// accumulator layout of v_mfma_f32_32x32x2_f32: lane l holds column l % 32,
-// rows 8 (r / 4) + 4 (l / 32) + r % 4).
-void tile_update(Checked& M, int c1, int k, int m, int r0, int c0, int kb)
-{
-    static Lds<float, kNB, kT + 1> sa, sb;
-    sa.violations = sb.violations = &M.violations;
-    for (int tid = 0; tid < kThreads; ++tid)
-        for (int e = tid; e < kNB * kT; e += kThreads) {
-            const int kk = e / kT, i = e % kT;
-            float va = 0.0f, vb = 0.0f;
-            if (kk < kb) {
-                if (r0 + i < m) va = M.A(c1 + r0 + i, k + kk);
-                if (c0 + i < m) vb = M.A(k + kk, c1 + c0 + i);
-            }
-            sa.at(kk, i) = va;
-            sb.at(kk, i) = vb;
-        }
-    for (int tid = 0; tid < kThreads; ++tid) {
-        const int lane = tid & 63, wave = tid >> 6, wm = wave & 1, wn = wave >> 1;
-        const int col = wn * 32 + (lane & 31);
-        const int gc = c0 + col;
-        if (gc >= m) continue;
-        for (int r = 0; r < 16; ++r) {
-            const int lr = wm * 32 + 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3);
-            float acc = 0.0f;
-            for (int kk = 0; kk < kNB; ++kk) acc = std::fma(sa.at(kk, lr), sb.at(kk, col), acc);
-            if (r0 + lr < m) M.A(c1 + r0 + lr, c1 + gc) -= acc;
-        }
-    }
-}
-
 using Panel = Lds<float, kLuMaxOrder, kNB + 1>;
+
+// A22 -= L21 U12 tile by tile (64 x 64, down each tile column, then the
+// next), the operands of tile t staged in buffer t % 2 of the panel array
+// (k-major, 2 x kNB x 65 floats per buffer); each tile's C block is read,
+// then written (the MFMA accumulator layout of v_mfma_f32_32x32x2_f32: lane
+// l holds column l % 32, rows 8 (r / 4) + 4 (l / 32) + r % 4).
+void tile_sweep(Checked& M, Panel& pnl, int c1, int k, int m, int kb)
+{
+    const int tiles = (m + kT - 1) / kT;
+    constexpr int kTileFloats = 2 * kNB * (kT + 1);
+    auto buf = [&](int t, int half, int kk, int i) -> float& {
+        const int idx = (t & 1) * kTileFloats + half * kNB * (kT + 1) + kk * (kT + 1) + i;
+        return pnl.at(idx / (kNB + 1), idx % (kNB + 1));
+    };
+    int t = 0;
+    for (int tj = 0; tj < tiles; ++tj)
+        for (int ti = 0; ti < tiles; ++ti, ++t) {
+            const int r0 = ti * kT, c0 = tj * kT;
+            for (int tid = 0; tid < kThreads; ++tid)
+                for (int q = 0; q < kNB * kT / kThreads; ++q) {
+                    const int e = tid + kThreads * q;
+                    const int kk = e / kT, i = e % kT;
+                    float va = 0.0f, vb = 0.0f;
+                    if (kk < kb) {
+                        if (r0 + i < m) va = M.A(c1 + r0 + i, k + kk);
+                        if (c0 + i < m) vb = M.A(k + kk, c1 + c0 + i);
+                    }
+                    buf(t, 0, kk, i) = va;
+                    buf(t, 1, kk, i) = vb;
+                }
+            for (int tid = 0; tid < kThreads; ++tid) {
+                const int lane = tid & 63, wave = tid >> 6, wm = wave & 1, wn = wave >> 1;
+                const int col = wn * 32 + (lane & 31);
+                const int gc = c0 + col;
+                if (gc >= m) continue;
+                for (int r = 0; r < 16; ++r) {
+                    const int lr = wm * 32 + 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3);
+                    if (r0 + lr >= m) continue;
+                    const float cv = M.A(c1 + r0 + lr, c1 + gc);
+                    float acc = 0.0f;
+                    for (int kk = 0; kk < kNB; ++kk)
+                        acc = std::fma(buf(t, 0, kk, lr), buf(t, 1, kk, col), acc);
+                    M.A(c1 + r0 + lr, c1 + gc) = cv - acc;
+                }
+            }
+        }
+}
 
 // The small size class: wave w takes tiles w, w + 4, ... of the 32-blocks of
 // the trailing matrix (column-major tile order), stages its operands in its
@@ -269,9 +285,7 @@ void lu_matrix(Checked& M, int n, int* info)
             tile_update_small(M, pnl, c1, k, mr, kb);
             continue;
         }
-        const int tiles = (mr + kT - 1) / kT;
-        for (int tj = 0; tj < tiles; ++tj)
-            for (int ti = 0; ti < tiles; ++ti) tile_update(M, c1, k, mr, ti * kT, tj * kT, kb);
+        tile_sweep(M, pnl, c1, k, mr, kb);
     }
 }
 
