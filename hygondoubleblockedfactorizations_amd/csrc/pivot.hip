@@ -263,6 +263,7 @@ struct LeafShared {
     T wv[kLeafWaves];
     int wi[kLeafWaves];
     unsigned int rec[kMaxG][Rec<T>::RW];  // payloads of every record of the column
+    unsigned int rec_sink[kLeafThreads];  // the first poll's out-of-range slots land here
     int abort;                            // a bounded wait expired
     int piv[LW];                          // local pivot row of each column (workgroup 0)
     T lb[LW][LW + 1];                     // workgroup 0: the leaf's pivot rows
@@ -395,15 +396,38 @@ __device__ __attribute__((always_inline)) inline bool leaf_step(
     }
 
     // 5. gather: each thread polls its own words of the G records until all
-    //    carry the column's epoch; one barrier joins the workgroup
+    //    carry the column's epoch; one barrier joins the workgroup.  The
+    //    first poll issues every slot's load without branches: a slot past
+    //    the records re-reads word idx - total (its own word, no shared hot
+    //    spot) into a per-thread sink, and only words not yet published are
+    //    polled again.
     {
         const int total = G * R::RW;
         unsigned int pending = 0;
+        {
+            unsigned long long v[NP];
 #pragma unroll
-        for (int q = 0; q < NP; ++q)
-            if (tid + q * kLeafThreads < total) pending |= 1u << q;
+            for (int q = 0; q < NP; ++q) {
+                const int idx = tid + q * kLeafThreads;
+                v[q] = xload64(slot + (idx < total ? idx : idx - total));
+            }
+#pragma unroll
+            for (int q = 0; q < NP; ++q) {
+                const int idx = tid + q * kLeafThreads;
+                const bool valid = idx < total;
+                unsigned int* dst = valid ? &sh.rec[0][0] + idx : &sh.rec_sink[tid];
+                *dst = static_cast<unsigned int>(v[q]);
+                pending |= (valid && static_cast<unsigned int>(v[q] >> 32) != epoch) ? (1u << q)
+                                                                                       : 0u;
+            }
+        }
         unsigned long long spins = 0;
         while (pending != 0) {
+            if (++spins > a.spin_limit) {
+                sh.abort = 1;
+                break;
+            }
+            __builtin_amdgcn_s_sleep(1);
             unsigned long long v[NP];
 #pragma unroll
             for (int q = 0; q < NP; ++q)
@@ -415,12 +439,6 @@ __device__ __attribute__((always_inline)) inline bool leaf_step(
                     pending &= ~(1u << q);
                 }
             }
-            if (pending == 0) break;
-            if (++spins > a.spin_limit) {
-                sh.abort = 1;
-                break;
-            }
-            __builtin_amdgcn_s_sleep(1);
         }
     }
     __syncthreads();
@@ -446,10 +464,17 @@ __device__ __attribute__((always_inline)) inline bool leaf_step(
     //    8 bits)
     T gb = T(-1);
     int gk = kIdxNone;
-    for (int t = lane; t < G; t += 64) {
-        const int idx = static_cast<int>(sh.rec[t][R::IDX]);
-        amax_pick(gb, gk, from_words<T>(&sh.rec[t][R::VAL]),
-                  idx == kIdxNone ? kIdxNone : (idx << 8) | t);
+    // one record per lane (two passes only for the variant sized beyond 64
+    // workgroups)
+    constexpr int kPasses = NP > (64 * R::RW + kLeafThreads - 1) / kLeafThreads ? 2 : 1;
+#pragma unroll
+    for (int ps = 0; ps < kPasses; ++ps) {
+        const int t = lane + 64 * ps;
+        if (t < G) {
+            const int idx = static_cast<int>(sh.rec[t][R::IDX]);
+            amax_pick(gb, gk, from_words<T>(&sh.rec[t][R::VAL]),
+                      idx == kIdxNone ? kIdxNone : (idx << 8) | t);
+        }
     }
     wave_argmax_fast(gb, gk);
     const bool none = gk == kIdxNone;
@@ -460,19 +485,30 @@ __device__ __attribute__((always_inline)) inline bool leaf_step(
 #pragma unroll
     for (int r = 0; r < RPT; ++r) l[r] = T(0);
     if (piv != T(0)) {
-        const bool recip = fabs(piv) >= sfmin<T>();
-        const T rcp = T(1) / piv;
 #pragma unroll
         for (int r = 0; r < RPT; ++r) {
             if (lab[r] == p) lab[r] = j;
             else if (lab[r] == j) lab[r] = p;
         }
+        // |pivot| >= sfmin: multiply by the reciprocal, else divide (xGETF2);
+        // the test is uniform, so only one of the two paths runs
+        if (fabs(piv) >= sfmin<T>()) {
+            const T rcp = T(1) / piv;
 #pragma unroll
-        for (int r = 0; r < RPT; ++r) {
-            const bool upd = lab[r] > j && lab[r] != kIdxNone;
-            const T lv = recip ? x[r][U] * rcp : x[r][U] / piv;
-            l[r] = upd ? lv : T(0);
-            if (upd) x[r][U] = lv;
+            for (int r = 0; r < RPT; ++r) {
+                const bool upd = lab[r] > j && lab[r] != kIdxNone;
+                const T lv = x[r][U] * rcp;
+                l[r] = upd ? lv : T(0);
+                if (upd) x[r][U] = lv;
+            }
+        } else {
+#pragma unroll
+            for (int r = 0; r < RPT; ++r) {
+                const bool upd = lab[r] > j && lab[r] != kIdxNone;
+                const T lv = x[r][U] / piv;
+                l[r] = upd ? lv : T(0);
+                if (upd) x[r][U] = lv;
+            }
         }
     } else if (g == 0 && tid == 0) {
         atomicCAS(a.info, 0, a.c0 + j + 1);

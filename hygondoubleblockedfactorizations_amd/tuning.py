@@ -34,8 +34,8 @@ TABLE = os.path.join(HERE, "tuning_table.json")
 BATCHED_TABLE = os.path.join(HERE, "tuning_batched.json")
 
 # Candidate grids (outer, inner) per routine; the outer-TRSM block of GETRF.
-POTRF_GRID = [(2048, 256), (4096, 256), (4096, 512), (8192, 256), (8192, 512), (8192, 1024),
-              (12288, 512), (16384, 512), (16384, 1024)]
+POTRF_GRID = [(1024, 256), (1536, 256), (2048, 128), (2048, 256), (4096, 256), (4096, 512),
+              (8192, 256), (8192, 512), (8192, 1024), (12288, 512), (16384, 512), (16384, 1024)]
 GETRF_GRID = [(4096, 2048), (4096, 4096), (6144, 3072), (8192, 2048), (8192, 4096),
               (16384, 4096)]
 TRSM_BLOCKS = [2048, 4096]
