@@ -37,11 +37,14 @@ FP32_PEAK_TFLOPS = 157.3      # MI355X dense FP32 (MFMA)
 # WRITE_SIZE runs of this script, scaled by the same counters' reading of a
 # device copy of known size (FETCH_SIZE x 2.0, WRITE_SIZE x 1.0).
 GEMM_TRAFFIC = {
-    # 256 x 256 NN kernel, 7 dispatches per GETRF: 8.77 GB read + 0.62 GB written
-    (32768, 8192, 256, 16384, 4096, "gemm_update_256"): 9.39e9,
+    # 256 x 256 NN kernel, 11 dispatches per GETRF: 5.362 GB read + 0.507 GB written
+    (32768, 2048, 256, 8192, 4096, "gemm_update_256"): 5.869e9,
+    # 256 x 256 NT lower-trapezoid kernel, 9 dispatches per POTRF: 8.395 GB + 1.192 GB
+    (32768, 2048, 256, 8192, 4096, "gemm_update_256_lower"): 9.587e9,
 }
 GEMM_TRAFFIC_SOURCE = ("calibrated, not live: rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE of "
-                       "this bench's GETRF step (profiles/getrf_n32768_pmc.md)")
+                       "this bench's POTRF and GETRF steps (profiles/getrf_n32768_pmc.md, "
+                       "profiles/potrf_n32768_pmc.md)")
 HBM_PEAK_GBS = 8000.0
 
 
@@ -828,7 +831,8 @@ def main():
     # trailing-update GEMMs (a few hundred launches per step), so its event
     # pairs do not perturb the step time.
     if not args.no_profile:
-        _lib.profile_begin(h, ["gemm_update", "gemm_update_256"])
+        _lib.profile_begin(h, ["gemm_update", "gemm_update_256", "gemm_update_lower",
+                               "gemm_update_256_lower"])
     for i in range(args.steps):
         for r in routines:
             work.copy_(inputs[r])
@@ -889,8 +893,13 @@ def main():
     kernel_names = {
         "gemm_update_256": "mifact::gemm_kernel<float,256,256,16,4,2,NN,full,vec> "
                            "(eight waves, MFMA 32x32x2 f32): GETRF trailing updates",
-        "gemm_update": "mifact::gemm_kernel<float,128,128,16,2,2,...> (four waves, MFMA "
-                       "32x32x2 f32): POTRF trapezoid and mid-size GETRF updates",
+        "gemm_update_256_lower": "mifact::gemm_kernel<float,256,256,16,4,2,NT,lower,vec> "
+                                 "(eight waves, MFMA 32x32x2 f32): POTRF outer SYRK + GEMM "
+                                 "trapezoids",
+        "gemm_update": "mifact::gemm_kernel<float,128,128,16,2,2,NN,full,vec> (four waves, "
+                       "MFMA 32x32x2 f32): mid-size GETRF updates",
+        "gemm_update_lower": "mifact::gemm_kernel<float,128,128,16,2,2,NT,lower,vec> (four "
+                             "waves, MFMA 32x32x2 f32): mid-size POTRF trapezoids",
     }
     ranked = sorted((k for k in kernel_names if dominant.get(k) and dominant[k]["ms"] > 0),
                     key=lambda k: -dominant[k]["ms"])

@@ -221,7 +221,8 @@ class StageStats(ctypes.Structure):
 
 
 STAGES = ["gemm", "potf2", "trsm_leaf", "getf2_leaf", "row_moves", "perm", "gemm_lowp",
-          "solve", "batched", "gemm_update", "gemm_update_256"]
+          "solve", "batched", "gemm_update", "gemm_update_256", "gemm_update_lower",
+          "gemm_update_256_lower"]
 
 
 def profile_begin(handle, stages=None):

@@ -28,6 +28,8 @@ enum Stage {
     kStageBatched,
     kStageGemmUpdate,
     kStageGemmUpdate256,
+    kStageGemmUpdateLower,
+    kStageGemmUpdate256Lower,
     kStageCount
 };
 

@@ -404,12 +404,14 @@ static void launch_cfg(mifact_handle h, GemmOp op, GemmShape shape, bool vec, bo
     const double mnk_c = (shape == kShapeLower)
         ? (static_cast<double>(a.M) * a.N - 0.5 * static_cast<double>(a.N) * (a.N - 1))
         : static_cast<double>(a.M) * a.N;
-    // trailing updates on the large tiles are their own stages (one per
-    // tile size: the dominant kernels of the factorizations); small and
-    // in-place launches stay "gemm"
+    // trailing updates on the large tiles are their own stages, one per tile
+    // size and shape (each one kernel: the dominant kernels of the
+    // factorizations); small and in-place launches stay "gemm"
+    const bool lower = shape == kShapeLower;
     const Stage stage = beta0 ? kStageGemm
-                      : BM >= 256 ? kStageGemmUpdate256
-                      : BM >= 128 ? kStageGemmUpdate : kStageGemm;
+                      : BM >= 256 ? (lower ? kStageGemmUpdate256Lower : kStageGemmUpdate256)
+                      : BM >= 128 ? (lower ? kStageGemmUpdateLower : kStageGemmUpdate)
+                                  : kStageGemm;
     StageScope scope(h, stage, 2.0 * mnk_c * a.K,
                      sizeof(T) * ((static_cast<double>(a.M) + a.N) * a.K + 2.0 * mnk_c));
     const dim3 grid(static_cast<unsigned>(blocks));

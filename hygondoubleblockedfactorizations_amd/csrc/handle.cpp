@@ -163,7 +163,8 @@ mifact_status mifact_profile_end(mifact_handle h, mifact_stage_stats* stats, int
     h->profiling = false;
     static const char* names[mifact::kStageCount] = {"gemm", "potf2", "trsm_leaf", "getf2_leaf",
                                                      "row_moves", "perm", "gemm_lowp", "solve",
-                                                     "batched", "gemm_update", "gemm_update_256"};
+                                                     "batched", "gemm_update", "gemm_update_256",
+                                                     "gemm_update_lower", "gemm_update_256_lower"};
     mifact_stage_stats acc[mifact::kStageCount];
     for (int s = 0; s < mifact::kStageCount; ++s) {
         std::snprintf(acc[s].name, sizeof(acc[s].name), "%s", names[s]);

@@ -104,8 +104,10 @@ const char* mifact_version(void);
 typedef struct mifact_stage_stats {
     char name[32];   /* stage: gemm, potf2, trsm_leaf, getf2_leaf, row_moves, perm,
                         gemm_lowp, solve, batched, gemm_update (128 x 128
-                        tile trailing-update GEMM), gemm_update_256 (256 x
-                        256 tiles) */
+                        tile trailing-update GEMM, full shape),
+                        gemm_update_256 (256 x 256 tiles, full shape),
+                        gemm_update_lower, gemm_update_256_lower (the same
+                        tiles, lower-trapezoid SYRK + GEMM updates) */
     int calls;       /* launches */
     double flops;    /* algorithmic flops of those launches */
     double bytes;    /* algorithmic HBM bytes of those launches */
@@ -115,8 +117,8 @@ typedef struct mifact_stage_stats {
 mifact_status mifact_profile_begin(mifact_handle handle);
 /* Records only the stages whose bit is set in stage_mask (bit order: gemm,
  * potf2, trsm_leaf, getf2_leaf, row_moves, perm, gemm_lowp, solve, batched,
- * gemm_update, gemm_update_256), so a timed run can bracket just the
- * dominant kernel. */
+ * gemm_update, gemm_update_256, gemm_update_lower, gemm_update_256_lower),
+ * so a timed run can bracket just the dominant kernel. */
 mifact_status mifact_profile_begin_stages(mifact_handle handle, unsigned stage_mask);
 /* Writes up to max_stages entries; *count receives the number of stages. */
 mifact_status mifact_profile_end(mifact_handle handle, mifact_stage_stats* stats,
