@@ -37,7 +37,7 @@ BATCHED_TABLE = os.path.join(HERE, "tuning_batched.json")
 POTRF_GRID = [(1024, 256), (1536, 256), (2048, 128), (2048, 256), (4096, 256), (4096, 512),
               (8192, 256), (8192, 512), (8192, 1024), (12288, 512), (16384, 512), (16384, 1024)]
 GETRF_GRID = [(4096, 2048), (4096, 4096), (6144, 3072), (8192, 2048), (8192, 4096),
-              (16384, 4096)]
+              (8192, 8192), (12288, 4096), (16384, 4096), (16384, 8192)]
 TRSM_BLOCKS = [2048, 4096]
 GEMM_POLICIES = [0, 1, 2, 3]   # mifact_set_gemm_policy
 # largest trailing order of the small size class (mifact_set_batched_tiles)
