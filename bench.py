@@ -37,10 +37,10 @@ FP32_PEAK_TFLOPS = 157.3      # MI355X dense FP32 (MFMA)
 # WRITE_SIZE runs of this script, scaled by the same counters' reading of a
 # device copy of known size (FETCH_SIZE x 2.0, WRITE_SIZE x 1.0).
 GEMM_TRAFFIC = {
-    # 256 x 256 NN kernel, 11 dispatches per GETRF: 5.362 GB read + 0.507 GB written
-    (32768, 2048, 256, 8192, 4096, "gemm_update_256"): 5.869e9,
-    # 256 x 256 NT lower-trapezoid kernel, 9 dispatches per POTRF: 8.395 GB + 1.192 GB
-    (32768, 2048, 256, 8192, 4096, "gemm_update_256_lower"): 9.587e9,
+    # 256 x 256 NN kernel, 8 dispatches per GETRF: 8.156 GB read + 0.625 GB written
+    (32768, 1024, 256, 16384, 8192, "gemm_update_256"): 8.781e9,
+    # 128 x 128 NT lower-trapezoid kernel, 50 dispatches per POTRF: 3.813 GB + 0.594 GB
+    (32768, 1024, 256, 16384, 8192, "gemm_update_lower"): 4.407e9,
 }
 GEMM_TRAFFIC_SOURCE = ("calibrated, not live: rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE of "
                        "this bench's POTRF and GETRF steps (profiles/getrf_n32768_pmc.md, "
@@ -76,10 +76,13 @@ def parse():
     ap.add_argument("--no-solve", action="store_true",
                     help="skip the mixed+IR vs FP32 solve comparison")
     ap.add_argument("--solve-precision", default="fp16", choices=["bf16", "fp16"])
+    ap.add_argument("--mixed-nb", type=int, default=2048,
+                    help="outer block of the mixed-precision Cholesky (n=32768, FP16, inner "
+                         "256: outer 2048 -> 50.5 ms, 1024 -> 53.3 ms, 8192 -> 57.9 ms); the "
+                         "mixed LU keeps the FP32 path's tuned outer block")
     ap.add_argument("--mixed-b", type=int, default=256,
-                    help="inner block of the mixed-precision factorization (outer block: the "
-                         "FP32 path's tuned one; n=32768, FP16, outer 2048: inner 256 -> "
-                         "50.5 ms, 512 -> 52.8 ms, 1024 -> 57.4 ms)")
+                    help="inner block of the mixed-precision factorization (n=32768, FP16, "
+                         "outer 2048: inner 256 -> 50.5 ms, 512 -> 52.8 ms, 1024 -> 57.4 ms)")
     ap.add_argument("--solve-kappa", type=float, default=1e4,
                     help="condition number of the SPD solve matrix Q diag(sigma) Q^T")
     ap.add_argument("--no-batched", action="store_true", help="skip the batched section")
@@ -270,7 +273,15 @@ def cpu_baseline(torch, args, n_default):
     }
 
 
-def solve_compare(torch, lib, h, P, n, routine, A, nb, b, tb, prec, seed, mixed_b):
+def mixed_blocks(args, routine, nb, b):
+    """(outer, inner) blocks of the mixed-precision factorization."""
+    if routine == "potrf":
+        mnb = args.mixed_nb if args.mixed_nb > 0 else nb
+        return mnb, max(1, min(args.mixed_b, mnb))
+    return nb, max(b, min(args.mixed_b, nb))
+
+
+def solve_compare(torch, lib, h, P, n, routine, A, nb, b, tb, prec, seed, mixed):
     """Time-to-solution of the plain FP32 solve (FP32 factorization + FP32
     triangular solves) against the mixed-precision factorization with FP64
     refinement run to the FP32 solve's own backward error."""
@@ -321,7 +332,7 @@ def solve_compare(torch, lib, h, P, n, routine, A, nb, b, tb, prec, seed, mixed_
     log(f"{routine} FP32 solve {t32:.1f} ms berr {berr32:.2e}")
     # mixed + FP64 refinement to the same backward error
     x = torch.zeros_like(b64)
-    prm = _lib.RefineParams(_lib.PRECISION[prec], nb, max(b, min(mixed_b, nb)), 1, tb, 30, berr32,
+    prm = _lib.RefineParams(_lib.PRECISION[prec], mixed[0], mixed[1], 1, tb, 30, berr32,
                             1, _lib.REFINE_METHOD["auto"])
     res = _lib.RefineResult()
     for rep in range(2):
@@ -340,7 +351,8 @@ def solve_compare(torch, lib, h, P, n, routine, A, nb, b, tb, prec, seed, mixed_
             raise RuntimeError(f"refine status {st}")
     berr_m = berr_of(x)
     fwd_m = ((x - xt).abs().max() / xt.abs().max()).item()
-    out["mixed_ir"] = dict(res.as_dict(), ms=tm, precision=prec, backward_error_check=berr_m,
+    out["mixed_ir"] = dict(res.as_dict(), ms=tm, precision=prec, nb=mixed[0], b=mixed[1],
+                           backward_error_check=berr_m,
                            forward_error=fwd_m, tolerance=berr32)
     out["speedup_time_to_solution"] = t32 / tm
     log(f"{routine} mixed {prec}+IR {tm:.1f} ms berr {berr_m:.2e} "
@@ -424,7 +436,7 @@ def fp64_reference(torch, routine, M, B):
     return X
 
 
-def sweep_case(torch, lib, h, P, routine, A, M64, nrhs, precisions, blocks, mixed_b, seed):
+def sweep_case(torch, lib, h, P, routine, A, M64, nrhs, precisions, blocks, mixed, seed):
     """One matrix and right-hand-side count of the solve sweep: the plain
     FP32 solve (factorization + triangular solves) and, per precision, the
     mixed factorization + FP64 refinement to the FP32 solve's backward
@@ -485,7 +497,7 @@ def sweep_case(torch, lib, h, P, routine, A, M64, nrhs, precisions, blocks, mixe
     del X32
     for prec in precisions:
         X = torch.zeros_like(B)
-        prm = _lib.RefineParams(_lib.PRECISION[prec], nb, max(b, min(mixed_b, nb)), 1, tb, 30,
+        prm = _lib.RefineParams(_lib.PRECISION[prec], mixed[0], mixed[1], 1, tb, 30,
                                 berr32, 1, _lib.REFINE_METHOD["auto"])
         res = _lib.RefineResult()
         for rep in range(2):
@@ -538,7 +550,8 @@ def solve_sweep(torch, lib, h, P, args):
             for nrhs in (1, 16):
                 log(f"sweep potrf n={n} kappa={kappa:g} nrhs={nrhs}")
                 case = sweep_case(torch, lib, h, P, "potrf", A, M, nrhs, ("bf16", "fp16"),
-                                  (pnb, pb, tb), args.mixed_b, seed + nrhs)
+                                  (pnb, pb, tb), mixed_blocks(args, "potrf", pnb, pb),
+                                  seed + nrhs)
                 case["matrix"] = f"Q diag(sigma) Q^T, kappa={kappa:g}"
                 out["potrf"].append(case)
             del A, M
@@ -564,7 +577,7 @@ def solve_sweep(torch, lib, h, P, args):
             M = A.t().double().contiguous()
             log(f"sweep getrf n={n} {label}")
             case = sweep_case(torch, lib, h, P, "getrf", A, M, 1, ("fp16", "fp8"),
-                              (gnb, gb, tb), args.mixed_b, seed + 7)
+                              (gnb, gb, tb), mixed_blocks(args, "getrf", gnb, gb), seed + 7)
             case["matrix"] = label
             out["getrf"].append(case)
             del A, M
@@ -942,13 +955,15 @@ def main():
                     log(f"kappa={args.solve_kappa:g} SPD matrix in {time.perf_counter() - t_gen:.1f} s")
                     lib.mifact_set_gemm_policy(h.ptr, policy[r])
                     solve[r] = solve_compare(torch, lib, h, P, n, r, A, pnb, pb, tb,
-                                             args.solve_precision, args.seed + 8, args.mixed_b)
+                                             args.solve_precision, args.seed + 8,
+                                             mixed_blocks(args, r, pnb, pb))
                     solve[r]["matrix"] = f"Q diag(sigma) Q^T, kappa={args.solve_kappa:g}"
                     del A
                 else:
                     lib.mifact_set_gemm_policy(h.ptr, policy[r])
                     solve[r] = solve_compare(torch, lib, h, P, n, r, inputs[r], gnb, gb, tb,
-                                             args.solve_precision, args.seed + 9, args.mixed_b)
+                                             args.solve_precision, args.seed + 9,
+                                             mixed_blocks(args, r, gnb, gb))
                     solve[r]["matrix"] = "iid Uniform(-1,1)"
         except Exception as e:  # report, never hide the GPU line
             log(f"solve comparison failed: {e}")
