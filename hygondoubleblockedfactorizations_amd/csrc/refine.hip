@@ -665,13 +665,18 @@ __global__ __launch_bounds__(256) void trsv_kernel(const float* F, long long ldf
     for (int e = tid; e < 64 * NR; e += 256) {
         const int r = e / NR, k = e % NR;
         if (r < rb && k < nr) {
-            T[off(r0 + r, k, ldt)] =
-                (part[0][r][k] + part[1][r][k]) + (part[2][r][k] + part[3][r][k]);
+            // device-scope store: written through to the device's point of
+            // coherence, no dirty L2 line left behind
+            __hip_atomic_store(T + off(r0 + r, k, ldt),
+                               (part[0][r][k] + part[1][r][k]) + (part[2][r][k] + part[3][r][k]),
+                               __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         }
     }
-    __threadfence();
+    // every wave's stores have completed at the device before the barrier,
+    // so the flag needs no L2 write-back (what a release fence would add)
+    __builtin_amdgcn_s_waitcnt(0);
     __syncthreads();
-    if (tid == 0) __hip_atomic_store(flags + bi, epoch, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+    if (tid == 0) __hip_atomic_store(flags + bi, epoch, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
 // ---------------------------------------------------------------- host pieces
