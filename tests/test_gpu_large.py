@@ -182,6 +182,41 @@ def to_host(buf):
     return np.asfortranarray(buf.cpu().numpy().T)
 
 
+@pytest.mark.parametrize("nb,b,policy", [(512, 128, 0), (1024, 256, 1), (1024, 256, 2)])
+def test_potrf_lookahead_deterministic(H, nb, b, policy):
+    """POTRF with many outer blocks, whose later blocks are updated on the
+    handle's side stream while the next block's panel runs on the main
+    stream: two runs on one handle and one on a fresh handle bound to
+    another stream give the same bits (every block receives its updates in
+    block order whatever the overlap), and the oracle residual is within the
+    1e-5 gate at n = 8192."""
+    import torch
+    from hygondoubleblockedfactorizations_amd import _lib
+    n = 8192
+    log("potrf lookahead", n, nb, b, policy)
+    assert lib().mifact_set_gemm_policy(H.ptr, policy) == 0
+    try:
+        Ab = spd_gpu(n, 31337)
+        L1, i1 = potrf_dev(H, Ab, nb, b)
+        L2, i2 = potrf_dev(H, Ab, nb, b)
+        s = torch.cuda.Stream()
+        h2 = _lib.Handle(stream=s)
+        assert lib().mifact_set_gemm_policy(h2.ptr, policy) == 0
+        L3 = Ab.clone()
+        info3 = torch.full((1,), 99, dtype=torch.int32, device="cuda")
+        torch.cuda.synchronize()
+        lib().double_blocked_potrf(h2.ptr, P(L3), n, n, nb, b, P(info3))
+        torch.cuda.synchronize()
+        assert h2.last_error() == 0
+        h2.close()
+        assert i1 == i2 == int(info3[0]) == 0
+        assert torch.equal(L1, L2) and torch.equal(L1, L3)
+        res = O.potrf_residual(to_host(L1), to_host(Ab))
+        assert res <= POTRF_GATE and res / (n * EPS32) <= 10, res
+    finally:
+        lib().mifact_set_gemm_policy(H.ptr, 0)
+
+
 @pytest.mark.parametrize("n", [8192, 16384])
 def test_potrf_sweep_sizes(H, n):
     """Benchmark-sweep POTRF at n = 8192 / 16384 with the autotuned blocks:
