@@ -352,9 +352,10 @@ constexpr int kPanelLd = kNB + 1;   // LDS row stride: conflict-free column walk
 //   1. the m x kb panel (m = n - k <= 1024 rows) is copied into LDS and
 //      factored there column by column (arg-max, row exchange within the
 //      panel, scaling, rank-1 update of the panel's remaining columns);
-//   2. the factored panel is written back; the panel's interchanges are
-//      applied to the columns left and right of it (one thread per column,
-//      in order);
+//   2. the factored panel is written back; the panel's interchanges,
+//      composed into one move list (at most 2 x 32 rows), are applied to
+//      the columns left and right of it (one thread per column: every moved
+//      row read, then written);
 //   3. U12 := L11^{-1} A12 (unit lower, L11 from LDS; one thread per column);
 //   4. A22 -= L21 U12 on MFMA tiles.
 // Every global access is to (row, column) of the n x n matrix with both
@@ -368,6 +369,8 @@ __device__ void lu_matrix(float* A, long long lda, int n, int* ipiv, int* info, 
     __shared__ float wv[4];
     __shared__ int wi[4];
     __shared__ int piv_s[kNB];
+    __shared__ int rowmap[kLuMaxOrder];         // panel rows: whose content ends where
+    __shared__ int mv_src[2 * kNB], mv_dst[2 * kNB];
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     constexpr float sfmin = FLT_MIN;
     for (int k = 0; k < n; k += kNB) {
@@ -444,16 +447,53 @@ __device__ void lu_matrix(float* A, long long lda, int n, int* ipiv, int* info, 
             const int r = e % m, c = e / m;
             A[off(k + r, k + c, lda)] = pnl[r][c];
         }
-        for (int t = tid; t < n - kb; t += 256) {
-            const int c = t < k ? t : t + kb;
+        // the panel's interchanges, composed: rowmap[r] = the row whose
+        // content ends at row r, for r in {0..kb-1} and the pivot rows
+        if (tid < kb) {
+            rowmap[tid] = tid;
+            rowmap[piv_s[tid]] = piv_s[tid];
+        }
+        __syncthreads();
+        if (tid == 0) {
             for (int jj = 0; jj < kb; ++jj) {
                 const int p = piv_s[jj];
                 if (p != jj) {
-                    const float x = A[off(k + jj, c, lda)];
-                    A[off(k + jj, c, lda)] = A[off(k + p, c, lda)];
-                    A[off(k + p, c, lda)] = x;
+                    const int t = rowmap[jj];
+                    rowmap[jj] = rowmap[p];
+                    rowmap[p] = t;
                 }
             }
+        }
+        __syncthreads();
+        // move list: entry t < kb for row t, entry kb + u for pivot row
+        // piv_s[u] below the panel's first kb rows (its first occurrence);
+        // src -1 where nothing moves
+        if (tid < 2 * kNB) {
+            int dst = -1;
+            if (tid < kb) {
+                dst = tid;
+            } else if (tid >= kNB && tid - kNB < kb) {
+                const int u = tid - kNB;
+                const int p = piv_s[u];
+                bool first = p >= kb;
+                for (int v = 0; v < u; ++v) first = first && piv_s[v] != p;
+                if (first) dst = p;
+            }
+            const int src = dst >= 0 ? rowmap[dst] : dst;
+            mv_dst[tid] = dst;
+            mv_src[tid] = (dst >= 0 && src != dst) ? src : -1;
+        }
+        __syncthreads();
+        // every other column: all moved rows read, then written
+        for (int t = tid; t < n - kb; t += 256) {
+            const int c = t < k ? t : t + kb;
+            float v[2 * kNB];
+#pragma unroll
+            for (int i = 0; i < 2 * kNB; ++i)
+                v[i] = mv_src[i] >= 0 ? A[off(k + mv_src[i], c, lda)] : 0.0f;
+#pragma unroll
+            for (int i = 0; i < 2 * kNB; ++i)
+                if (mv_src[i] >= 0) A[off(k + mv_dst[i], c, lda)] = v[i];
         }
         __syncthreads();
         const int c1 = k + kb;

@@ -178,6 +178,9 @@ void lu_matrix(Checked& M, int n, int* info)
 {
     static Panel pnl;
     Lds<int, 1, kNB> piv_s;
+    static Lds<int, 1, kLuMaxOrder> rowmap;
+    Lds<int, 1, 2 * kNB> mv_src, mv_dst;
+    rowmap.violations = mv_src.violations = mv_dst.violations = &M.violations;
     pnl.violations = piv_s.violations = &M.violations;
     for (int k = 0; k < n; k += kNB) {
         const int kb = std::min(kNB, n - k);
@@ -258,12 +261,39 @@ void lu_matrix(Checked& M, int n, int* info)
             const int r = e % m, c = e / m;
             M.A(k + r, k + c) = pnl.at(r, c);
         }
+        // the panel's interchanges composed into a move list (rowmap: whose
+        // content ends at each panel row), then every other column: all
+        // moved rows read, then written
+        for (int tid = 0; tid < kb; ++tid) {
+            rowmap.at(0, tid) = tid;
+            rowmap.at(0, piv_s.at(0, tid)) = piv_s.at(0, tid);
+        }
+        for (int jj = 0; jj < kb; ++jj) {
+            const int pp = piv_s.at(0, jj);
+            if (pp != jj) std::swap(rowmap.at(0, jj), rowmap.at(0, pp));
+        }
+        for (int tid = 0; tid < 2 * kNB; ++tid) {
+            int dst = -1;
+            if (tid < kb) {
+                dst = tid;
+            } else if (tid >= kNB && tid - kNB < kb) {
+                const int u = tid - kNB;
+                const int pp = piv_s.at(0, u);
+                bool first = pp >= kb;
+                for (int v = 0; v < u; ++v) first = first && piv_s.at(0, v) != pp;
+                if (first) dst = pp;
+            }
+            const int src = dst >= 0 ? rowmap.at(0, dst) : dst;
+            mv_dst.at(0, tid) = dst;
+            mv_src.at(0, tid) = (dst >= 0 && src != dst) ? src : -1;
+        }
         for (int t = 0; t < n - kb; ++t) {
             const int c = t < k ? t : t + kb;
-            for (int jj = 0; jj < kb; ++jj) {
-                const int pp = piv_s.at(0, jj);
-                if (pp != jj) std::swap(M.A(k + jj, c), M.A(k + pp, c));
-            }
+            float v[2 * kNB];
+            for (int i = 0; i < 2 * kNB; ++i)
+                v[i] = mv_src.at(0, i) >= 0 ? M.A(k + mv_src.at(0, i), c) : 0.0f;
+            for (int i = 0; i < 2 * kNB; ++i)
+                if (mv_src.at(0, i) >= 0) M.A(k + mv_dst.at(0, i), c) = v[i];
         }
         const int c1 = k + kb, mr = n - c1;
         if (mr <= 0) continue;
