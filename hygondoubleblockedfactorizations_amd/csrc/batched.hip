@@ -105,14 +105,16 @@ __device__ void tile_sweep(float* C, long long ldc, int m, int kb, LA la, RB rb,
         const bool more = nj < tiles;
         const int row = wm * 32 + (lane & 31);
         const int col = wn * 32 + (lane & 31);
-        // this tile's C block (accumulator r: row 8 (r / 4) + 4 (lane / 32) + r % 4),
-        // read before the next tile's operands: loads complete in issue
-        // order, so the store below waits for these only
-        const int gc = c0 + col;
+        // this tile's C block, read before the next tile's operands (loads
+        // complete in issue order, so the store below waits for these only).
+        // The products run as C^T -= R^T L^T, so accumulator r holds C row
+        // `gr` (the lane's, consecutive down a column: two 128-byte lines
+        // per load) and column 8 (r / 4) + 4 (lane / 32) + r % 4
+        const int gr = r0 + row;
         float cv[16];
 #pragma unroll
         for (int r = 0; r < 16; ++r) {
-            const int gr = r0 + wm * 32 + 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3);
+            const int gc = c0 + wn * 32 + 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3);
             const bool ok = gc < m && gr < m && (!LOWER || gr >= gc);
             cv[r] = ok ? C[off(gr, gc, ldc)] : 0.0f;
         }
@@ -125,7 +127,7 @@ __device__ void tile_sweep(float* C, long long ldc, int m, int kb, LA la, RB rb,
             for (int kk = 0; kk < kNB; kk += 2) {
                 const float a = sa[kk + (lane >> 5)][row];
                 const float b = sb[kk + (lane >> 5)][col];
-                acc = __builtin_amdgcn_mfma_f32_32x32x2f32(a, b, acc, 0, 0, 0);
+                acc = __builtin_amdgcn_mfma_f32_32x32x2f32(b, a, acc, 0, 0, 0);
             }
         } else {
 #pragma unroll
@@ -138,7 +140,7 @@ __device__ void tile_sweep(float* C, long long ldc, int m, int kb, LA la, RB rb,
                         a[e] = static_cast<__bf16>(sa[k0 + e][row]);
                         b[e] = static_cast<__bf16>(sb[k0 + e][col]);
                     }
-                    acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, b, acc, 0, 0, 0);
+                    acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(b, a, acc, 0, 0, 0);
                 } else {
                     halfx8 a, b;
 #pragma unroll
@@ -146,13 +148,13 @@ __device__ void tile_sweep(float* C, long long ldc, int m, int kb, LA la, RB rb,
                         a[e] = static_cast<_Float16>(sa[k0 + e][row]);
                         b[e] = static_cast<_Float16>(sb[k0 + e][col]);
                     }
-                    acc = __builtin_amdgcn_mfma_f32_32x32x16_f16(a, b, acc, 0, 0, 0);
+                    acc = __builtin_amdgcn_mfma_f32_32x32x16_f16(b, a, acc, 0, 0, 0);
                 }
             }
         }
 #pragma unroll
         for (int r = 0; r < 16; ++r) {
-            const int gr = r0 + wm * 32 + 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3);
+            const int gc = c0 + wn * 32 + 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3);
             if (gc < m && gr < m && (!LOWER || gr >= gc)) C[off(gr, gc, ldc)] = cv[r] - acc[r];
         }
         if (!more) break;
@@ -212,8 +214,8 @@ __device__ void tile_update_small(float* C, long long ldc, int m, int kb, LA la,
         if (PREC == kPrecFP32) {
 #pragma unroll
             for (int kk = 0; kk < kNB; kk += 2)
-                acc = __builtin_amdgcn_mfma_f32_32x32x2f32(sa[kk + (lane >> 5)][rc],
-                                                            sb[kk + (lane >> 5)][rc], acc, 0, 0, 0);
+                acc = __builtin_amdgcn_mfma_f32_32x32x2f32(sb[kk + (lane >> 5)][rc],
+                                                            sa[kk + (lane >> 5)][rc], acc, 0, 0, 0);
         } else {
 #pragma unroll
             for (int kk = 0; kk < kNB; kk += 16) {
@@ -225,7 +227,7 @@ __device__ void tile_update_small(float* C, long long ldc, int m, int kb, LA la,
                         a[e] = static_cast<__bf16>(sa[k0 + e][rc]);
                         b[e] = static_cast<__bf16>(sb[k0 + e][rc]);
                     }
-                    acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, b, acc, 0, 0, 0);
+                    acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(b, a, acc, 0, 0, 0);
                 } else {
                     halfx8 a, b;
 #pragma unroll
@@ -233,17 +235,18 @@ __device__ void tile_update_small(float* C, long long ldc, int m, int kb, LA la,
                         a[e] = static_cast<_Float16>(sa[k0 + e][rc]);
                         b[e] = static_cast<_Float16>(sb[k0 + e][rc]);
                     }
-                    acc = __builtin_amdgcn_mfma_f32_32x32x16_f16(a, b, acc, 0, 0, 0);
+                    acc = __builtin_amdgcn_mfma_f32_32x32x16_f16(b, a, acc, 0, 0, 0);
                 }
             }
         }
-        // accumulator r: row 8 (r / 4) + 4 (lane / 32) + r % 4, column lane % 32
-        const int gc = c0 + rc;
-        if (gc < m) {
+        // C^T -= R^T L^T: accumulator r holds C row rc (the lane's),
+        // column 8 (r / 4) + 4 (lane / 32) + r % 4
+        const int gr = r0 + rc;
+        if (gr < m) {
 #pragma unroll
             for (int r = 0; r < 16; ++r) {
-                const int gr = r0 + 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3);
-                if (gr < m && (!LOWER || gr >= gc)) C[off(gr, gc, ldc)] -= acc[r];
+                const int gc = c0 + 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3);
+                if (gc < m && (!LOWER || gr >= gc)) C[off(gr, gc, ldc)] -= acc[r];
             }
         }
         // the wave's slice is restaged by its next tile

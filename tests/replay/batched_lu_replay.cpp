@@ -91,8 +91,9 @@ using Panel = Lds<float, kLuMaxOrder, kNB + 1>;
 // A22 -= L21 U12 tile by tile (64 x 64, down each tile column, then the
 // next), the operands of tile t staged in buffer t % 2 of the panel array
 // (k-major, 2 x kNB x 65 floats per buffer); each tile's C block is read,
-// then written (the MFMA accumulator layout of v_mfma_f32_32x32x2_f32: lane
-// l holds column l % 32, rows 8 (r / 4) + 4 (l / 32) + r % 4).
+// then written (the products run as C^T -= R^T L^T, so in the MFMA
+// accumulator layout lane l holds row l % 32, columns 8 (r / 4) + 4 (l / 32)
+// + r % 4).
 void tile_sweep(Checked& M, Panel& pnl, int c1, int k, int m, int kb)
 {
     const int tiles = (m + kT - 1) / kT;
@@ -119,17 +120,17 @@ void tile_sweep(Checked& M, Panel& pnl, int c1, int k, int m, int kb)
                 }
             for (int tid = 0; tid < kThreads; ++tid) {
                 const int lane = tid & 63, wave = tid >> 6, wm = wave & 1, wn = wave >> 1;
-                const int col = wn * 32 + (lane & 31);
-                const int gc = c0 + col;
-                if (gc >= m) continue;
+                const int row = wm * 32 + (lane & 31);
+                const int gr = r0 + row;
+                if (gr >= m) continue;
                 for (int r = 0; r < 16; ++r) {
-                    const int lr = wm * 32 + 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3);
-                    if (r0 + lr >= m) continue;
-                    const float cv = M.A(c1 + r0 + lr, c1 + gc);
+                    const int lc = wn * 32 + 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3);
+                    if (c0 + lc >= m) continue;
+                    const float cv = M.A(c1 + gr, c1 + c0 + lc);
                     float acc = 0.0f;
                     for (int kk = 0; kk < kNB; ++kk)
-                        acc = std::fma(buf(t, 0, kk, lr), buf(t, 1, kk, col), acc);
-                    M.A(c1 + r0 + lr, c1 + gc) = cv - acc;
+                        acc = std::fma(buf(t, 1, kk, lc), buf(t, 0, kk, row), acc);
+                    M.A(c1 + gr, c1 + c0 + lc) = cv - acc;
                 }
             }
         }
@@ -161,14 +162,14 @@ void tile_update_small(Checked& M, Panel& pnl, int c1, int k, int m, int kb)
                     slot(wave, 1, kk, i) = vb;
                 }
             for (int lane = 0; lane < 64; ++lane) {
-                const int col = lane & 31, gc = c0 + col;
-                if (gc >= m) continue;
+                const int rc = lane & 31, gr = r0 + rc;
+                if (gr >= m) continue;
                 for (int r = 0; r < 16; ++r) {
-                    const int lr = 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3);
+                    const int lc = 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3);
                     float acc = 0.0f;
                     for (int kk = 0; kk < kNB; ++kk)
-                        acc = std::fma(slot(wave, 0, kk, lr), slot(wave, 1, kk, col), acc);
-                    if (r0 + lr < m) M.A(c1 + r0 + lr, c1 + gc) -= acc;
+                        acc = std::fma(slot(wave, 1, kk, lc), slot(wave, 0, kk, rc), acc);
+                    if (c0 + lc < m) M.A(c1 + gr, c1 + c0 + lc) -= acc;
                 }
             }
         }
