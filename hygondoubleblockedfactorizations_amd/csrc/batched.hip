@@ -259,86 +259,116 @@ __device__ void tile_update_small(float* C, long long ldc, int m, int kb, LA la,
 
 // ---------------------------------------------------------------- Cholesky
 
+__device__ inline float lane_value(float v, int lane)
+{
+    return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), lane));
+}
+
+// Right-looking lower Cholesky of one matrix by one workgroup (LAPACK
+// spotrf semantics: info = the first column whose pivot is not positive and
+// finite; the block being factored is written back as far as it got).
+// Per 32-column panel:
+//   1. POTF2 of the diagonal block on wave 0 alone, lane i holding row i in
+//      registers (pivots and column entries broadcast with readlane, no
+//      barrier); L11 and 1 / diag(L11) go to LDS, L11 back to global memory;
+//   2. TRSM A21 := A21 L11^{-T}, one row per thread in registers, L11 read
+//      as workgroup-uniform LDS broadcasts;
+//   3. A22 (lower) -= A21 A21^T on the MFMA tiles.
+// A last panel narrower than 32 (kb < 32) reads its column kb - 1 (and, in
+// the POTF2, its row kb - 1) in place of the missing ones: every access
+// stays inside the matrix and nothing computed from the copies is stored.
 template <int PREC>
 __device__ void chol_matrix(float* A, long long lda, int n, int* info, int small_max)
 {
-    __shared__ float d[kNB][kNB + 1];
-    __shared__ float xs[256][kNB + 1];   // TRSM row slots; the trailing update's operand buffers
-    static_assert(256 * (kNB + 1) >= kSmallScratch, "small-tile staging inside the row slots");
-    static_assert(256 * (kNB + 1) >= 2 * kTileFloats, "tile buffers inside the row slots");
+    __shared__ __align__(16) float lt[kNB][kNB];   // L11 (row-major; entries above the diagonal unused)
+    __shared__ float rdiag[kNB];                   // 1 / L(j, j)
+    constexpr int kScratch = kSmallScratch > 2 * kTileFloats ? kSmallScratch : 2 * kTileFloats;
+    __shared__ float scratch[kScratch];            // the trailing update's operand buffers
     __shared__ int failed;
     const int tid = threadIdx.x;
-    if (tid == 0) failed = 0;
     for (int k = 0; k < n; k += kNB) {
         const int kb = imin(kNB, n - k);
-        // diagonal block -> LDS (lower)
-        for (int e = tid; e < kNB * kNB; e += 256) {
-            const int i = e % kNB, j = e / kNB;
-            d[i][j] = (i < kb && j < kb && i >= j) ? A[off(k + i, k + j, lda)] : 0.0f;
-        }
-        __syncthreads();
-        // POTF2 (right-looking within the block)
-        for (int j = 0; j < kb; ++j) {
-            const float djj = d[j][j];
-            const bool bad = !(djj > 0.0f) || !isfinite(djj);
-            if (bad) {
-                if (tid == 0) {
-                    failed = 1;
-                    *info = k + j + 1;
+        // 1. POTF2
+        if (tid < 64) {
+            const int lane = tid;
+            // the row walked column by column with one pointer (precomputed
+            // per-column addresses would hold 64 registers)
+            const float* p = A + off(k + imin(lane, kb - 1), k, lda);
+            float x[kNB];
+#pragma unroll
+            for (int c = 0; c < kNB; ++c) {
+                x[c] = *p;
+                if (c + 1 < kb) p += lda;
+            }
+            int bad = -1;
+#pragma unroll
+            for (int j = 0; j < kNB; ++j) {
+                if (j >= kb) break;
+                const float djj = lane_value(x[j], j);
+                if (!(djj > 0.0f) || !isfinite(djj)) {
+                    bad = j;
+                    break;
                 }
-                break;
+                const float s = sqrtf(djj);
+                x[j] = lane == j ? s : (lane > j ? x[j] / s : x[j]);
+#pragma unroll
+                for (int c = j + 1; c < kNB; ++c) x[c] -= x[j] * lane_value(x[j], c);
             }
-            const float s = sqrtf(djj);
-            __syncthreads();
-            if (tid < kb) {
-                if (tid == j) d[j][j] = s;
-                else if (tid > j) d[tid][j] = d[tid][j] / s;
+            if (lane < kNB) {
+#pragma unroll
+                for (int c = 0; c < kNB; c += 4)
+                    *reinterpret_cast<float4*>(&lt[lane][c]) = make_float4(x[c], x[c + 1], x[c + 2], x[c + 3]);
+                rdiag[lane] = 1.0f / x[lane];
             }
-            __syncthreads();
-            for (int e = tid; e < kNB * kNB; e += 256) {
-                const int i = e % kNB, c = e / kNB;
-                if (c > j && i >= c && i < kb) d[i][c] -= d[i][j] * d[c][j];
+            if (tid == 0) {
+                failed = bad >= 0;
+                if (bad >= 0) *info = k + bad + 1;
             }
-            __syncthreads();
         }
         __syncthreads();
-        // write back the factored block (the columns done before a failure)
+        // the factored block (lower) back to global memory
         for (int e = tid; e < kNB * kNB; e += 256) {
-            const int i = e % kNB, j = e / kNB;
-            if (i < kb && j < kb && i >= j) A[off(k + i, k + j, lda)] = d[i][j];
+            const int i = e % kNB, c = e / kNB;
+            if (i < kb && c <= i) A[off(k + i, k + c, lda)] = lt[i][c];
         }
-        __syncthreads();
         if (failed) return;
         const int r1 = k + kb;
         const int m = n - r1;
         if (m <= 0) break;
-        // TRSM: A(r, k:k+kb) := A(r, k:k+kb) L^{-T}, one thread per row, the
-        // row in the thread's own LDS slot (runtime loops: a register-resident
-        // row with unrolled loops spills)
-        for (int r0 = r1; r0 < n; r0 += 256) {
-            const int r = r0 + tid;
-            float* x = xs[tid];
-            if (r < n) {
-                for (int j = 0; j < kb; ++j) x[j] = A[off(r, k + j, lda)];
-                for (int j = 0; j < kb; ++j) {
-                    float v = x[j];
-                    for (int c = 0; c < j; ++c) v -= x[c] * d[j][c];
-                    x[j] = v / d[j][j];
-                }
-                for (int j = 0; j < kb; ++j) A[off(r, k + j, lda)] = x[j];
+        // 2. TRSM
+        for (int r = r1 + tid; r < n; r += 256) {
+            const float* p = A + off(r, k, lda);
+            float x[kNB];
+#pragma unroll
+            for (int j = 0; j < kNB; ++j) {
+                x[j] = *p;
+                if (j + 1 < kb) p += lda;
+            }
+#pragma unroll
+            for (int j = 0; j < kNB; ++j) {
+                float v = x[j];
+#pragma unroll
+                for (int c = 0; c < j; ++c) v -= x[c] * lt[j][c];
+                x[j] = v * rdiag[j];
+                // row j + 1 of L11 is read after x[j] is done, not all 528
+                // entries ahead of the arithmetic (they would spill)
+                asm volatile("" : "+v"(x[j]) : : "memory");
+            }
+            float* q = A + off(r, k, lda);
+#pragma unroll
+            for (int j = 0; j < kNB; ++j) {
+                if (j < kb) *q = x[j];
+                if (j + 1 < kb) q += lda;
             }
         }
         __syncthreads();
-        // SYRK: A22 (lower) -= P P^T with P = A(r1:n, k:k+kb)
+        // 3. SYRK: A22 (lower) -= P P^T with P = A(r1:n, k:k+kb)
         float* C = A + off(r1, r1, lda);
         const float* Pp = A + off(r1, k, lda);
         auto la = [&](int i, int kk) { return Pp[off(i, kk, lda)]; };
         auto rb = [&](int kk, int j) { return Pp[off(j, kk, lda)]; };
-        if (m <= small_max) {
-            tile_update_small<PREC, true>(C, lda, m, kb, la, rb, &xs[0][0]);
-            continue;
-        }
-        tile_sweep<PREC, true>(C, lda, m, kb, la, rb, &xs[0][0]);
+        if (m <= small_max) tile_update_small<PREC, true>(C, lda, m, kb, la, rb, scratch);
+        else tile_sweep<PREC, true>(C, lda, m, kb, la, rb, scratch);
     }
 }
 
