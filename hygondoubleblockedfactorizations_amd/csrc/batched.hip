@@ -301,18 +301,21 @@ __device__ void chol_matrix(float* A, long long lda, int n, int* info, int small
                 if (c + 1 < kb) p += lda;
             }
             int bad = -1;
+            // fully unrolled (x[] stays in registers): the column loop has
+            // no early exit, later columns are skipped by uniform branches
 #pragma unroll
             for (int j = 0; j < kNB; ++j) {
-                if (j >= kb) break;
-                const float djj = lane_value(x[j], j);
-                if (!(djj > 0.0f) || !isfinite(djj)) {
-                    bad = j;
-                    break;
-                }
-                const float s = sqrtf(djj);
-                x[j] = lane == j ? s : (lane > j ? x[j] / s : x[j]);
+                if (j < kb && bad < 0) {
+                    const float djj = lane_value(x[j], j);
+                    if (!(djj > 0.0f) || !isfinite(djj)) {
+                        bad = j;
+                    } else {
+                        const float s = sqrtf(djj);
+                        x[j] = lane == j ? s : (lane > j ? x[j] / s : x[j]);
 #pragma unroll
-                for (int c = j + 1; c < kNB; ++c) x[c] -= x[j] * lane_value(x[j], c);
+                        for (int c = j + 1; c < kNB; ++c) x[c] -= x[j] * lane_value(x[j], c);
+                    }
+                }
             }
             if (lane < kNB) {
 #pragma unroll
