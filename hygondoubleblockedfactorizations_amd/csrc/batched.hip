@@ -51,41 +51,50 @@ struct BatchArgs {
     int batch;
 };
 
-// The 64 x 64 tiles of a trailing update C -= L R, one after another on the
-// workgroup's four waves (2 x 2 blocks of v_mfma_f32_32x32x*), software
-// pipelined: while tile t's products run, the operands of tile t + 1 are
-// read from global memory into registers, and tile t's block of C is read
-// before its products; two operand buffers (k-major, 2 x kNB x 65 floats
-// each) alternate in `scratch`, one workgroup barrier per tile.
+// The 64 x 64 tiles of a trailing update C -= L R (C m x nc, K = kb), one
+// after another on the workgroup's four waves (2 x 2 blocks of
+// v_mfma_f32_32x32x*), each tile in ceil(kb / 32) steps of 32-wide k slices
+// that accumulate before its block of C is written once.  Software
+// pipelined: while step s's products run, the operands of step s + 1 are
+// read from global memory into registers, and a tile's block of C is read
+// at its first step; two operand buffers (k-major, 2 x kNB x 65 floats
+// each) alternate in `scratch`, one workgroup barrier per step.
 // la(i, k): left operand element (row i); rb(k, j): right operand element
 // (column j).  LOWER: only the tiles and elements with row >= column.
+// MAXSL: the largest slice count the caller passes (kb <= MAXSL * 32); 1
+// compiles the single-slice sweep.
 constexpr int kTileFloats = 2 * kNB * (kT + 1);
 
-template <int PREC, bool LOWER, typename LA, typename RB>
-__device__ void tile_sweep(float* C, long long ldc, int m, int kb, LA la, RB rb, float* scratch)
+template <int PREC, bool LOWER, int MAXSL, typename LA, typename RB>
+__device__ void tile_sweep(float* C, long long ldc, int m, int nc, int kb, LA la, RB rb, float* scratch)
 {
     const int tid = threadIdx.x;
     const int lane = tid & 63, wave = tid >> 6;
     const int wm = wave & 1, wn = wave >> 1;
-    const int tiles = (m + kT - 1) / kT;
+    const int tiles_r = (m + kT - 1) / kT, tiles_c = (nc + kT - 1) / kT;
+    const int slices = MAXSL == 1 ? 1 : (kb + kNB - 1) / kNB;
     constexpr int PER = kNB * kT / 256;   // operand elements per thread and operand
     float na[PER], nbv[PER];
-    auto fetch = [&](int r0, int c0) {
+    auto fetch = [&](int r0, int c0, int k0) {
 #pragma unroll
         for (int q = 0; q < PER; ++q) {
             const int e = tid + 256 * q;
-            const int k = e / kT, i = e % kT;
+            const int k = k0 + e / kT, i = e % kT;
             float va = 0.0f, vb = 0.0f;
             if (k < kb) {
                 if (r0 + i < m) va = la(r0 + i, k);
-                if (c0 + i < m) vb = rb(k, c0 + i);
+                if (c0 + i < nc) vb = rb(k, c0 + i);
             }
             na[q] = va;
             nbv[q] = vb;
         }
     };
-    int ti = 0, tj = 0;
-    fetch(0, 0);
+    int ti = 0, tj = 0, sl = 0;
+    fetch(0, 0, 0);
+    const int row = wm * 32 + (lane & 31);
+    const int col = wn * 32 + (lane & 31);
+    float cv[16];
+    floatx16 acc;
     for (int t = 0;; ++t) {
         float (*sa)[kT + 1] = reinterpret_cast<float (*)[kT + 1]>(scratch + (t & 1) * kTileFloats);
         float (*sb)[kT + 1] = sa + kNB;
@@ -97,31 +106,33 @@ __device__ void tile_sweep(float* C, long long ldc, int m, int kb, LA la, RB rb,
         }
         __syncthreads();
         const int r0 = ti * kT, c0 = tj * kT;
-        int ni = ti + 1, nj = tj;               // next tile: down the column, then the next one
-        if (ni >= tiles) {
-            ++nj;
-            ni = LOWER ? nj : 0;
+        // next step: the next k slice, else down the tile column, then the next one
+        int nsl = sl + 1, ni = ti, nj = tj;
+        if (nsl == slices) {
+            nsl = 0;
+            ++ni;
+            if (ni >= tiles_r) {
+                ++nj;
+                ni = LOWER ? nj : 0;
+            }
         }
-        const bool more = nj < tiles;
-        const int row = wm * 32 + (lane & 31);
-        const int col = wn * 32 + (lane & 31);
-        // this tile's C block, read before the next tile's operands (loads
-        // complete in issue order, so the store below waits for these only).
-        // The products run as C^T -= R^T L^T, so accumulator r holds C row
+        const bool more = nj < tiles_c;
+        // the products run as C^T -= R^T L^T, so accumulator r holds C row
         // `gr` (the lane's, consecutive down a column: two 128-byte lines
         // per load) and column 8 (r / 4) + 4 (lane / 32) + r % 4
         const int gr = r0 + row;
-        float cv[16];
+        if (sl == 0) {
+            // this tile's C block, read before the next step's operands
+            // (loads complete in issue order, so the store waits for these only)
 #pragma unroll
-        for (int r = 0; r < 16; ++r) {
-            const int gc = c0 + wn * 32 + 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3);
-            const bool ok = gc < m && gr < m && (!LOWER || gr >= gc);
-            cv[r] = ok ? C[off(gr, gc, ldc)] : 0.0f;
+            for (int r = 0; r < 16; ++r) {
+                const int gc = c0 + wn * 32 + 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3);
+                const bool ok = gc < nc && gr < m && (!LOWER || gr >= gc);
+                cv[r] = ok ? C[off(gr, gc, ldc)] : 0.0f;
+                acc[r] = 0.0f;
+            }
         }
-        if (more) fetch(ni * kT, nj * kT);
-        floatx16 acc;
-#pragma unroll
-        for (int r = 0; r < 16; ++r) acc[r] = 0.0f;
+        if (more) fetch(ni * kT, nj * kT, nsl * kNB);
         if (PREC == kPrecFP32) {
 #pragma unroll
             for (int kk = 0; kk < kNB; kk += 2) {
@@ -152,14 +163,17 @@ __device__ void tile_sweep(float* C, long long ldc, int m, int kb, LA la, RB rb,
                 }
             }
         }
+        if (sl == slices - 1) {
 #pragma unroll
-        for (int r = 0; r < 16; ++r) {
-            const int gc = c0 + wn * 32 + 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3);
-            if (gc < m && gr < m && (!LOWER || gr >= gc)) C[off(gr, gc, ldc)] = cv[r] - acc[r];
+            for (int r = 0; r < 16; ++r) {
+                const int gc = c0 + wn * 32 + 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3);
+                if (gc < nc && gr < m && (!LOWER || gr >= gc)) C[off(gr, gc, ldc)] = cv[r] - acc[r];
+            }
         }
         if (!more) break;
         ti = ni;
         tj = nj;
+        sl = nsl;
     }
     // every wave is done with the buffers and with C before the caller goes on
     __syncthreads();
@@ -167,77 +181,93 @@ __device__ void tile_sweep(float* C, long long ldc, int m, int kb, LA la, RB rb,
 
 // Size class of the small trailing updates: one 32 x 32 tile per wave, four
 // tiles per pass, no workgroup barrier inside a pass.  Each wave stages its
-// operands (32 x kb of each, k-major) in its own 2 x 32 x 33 slice of
-// `scratch` (4 x 2112 floats).  Tiles t = 0 .. count - 1 in row-major order
-// over the lower triangle (LOWER) or the full square of 32-blocks.
+// operands one 32-wide k slice at a time (32 x 32 of each, k-major) in its
+// own 2 x 32 x 33 slice of `scratch` (4 x 2112 floats).  Tiles t = 0 ..
+// count - 1 in row-major order over the lower trapezoid (LOWER) or the
+// full rectangle of 32-blocks of the m x nc block C.
 constexpr int kSmallT = 32;
 constexpr int kSmallScratch = 4 * 2 * kNB * (kSmallT + 1);
 
 template <int PREC, bool LOWER, typename LA, typename RB>
-__device__ void tile_update_small(float* C, long long ldc, int m, int kb, LA la, RB rb,
+__device__ void tile_update_small(float* C, long long ldc, int m, int nc, int kb, LA la, RB rb,
                                   float* scratch)
 {
     const int tid = threadIdx.x;
     const int lane = tid & 63, wave = tid >> 6;
     float (*sa)[kSmallT + 1] = reinterpret_cast<float (*)[kSmallT + 1]>(scratch + wave * 2 * kNB * (kSmallT + 1));
     float (*sb)[kSmallT + 1] = sa + kNB;
-    const int tb = (m + kSmallT - 1) / kSmallT;
-    const int count = LOWER ? tb * (tb + 1) / 2 : tb * tb;
+    const int tr = (m + kSmallT - 1) / kSmallT, tc = (nc + kSmallT - 1) / kSmallT;
+    // LOWER: row-major over the trapezoid ti >= tj (tj < tc): row ti holds
+    // min(ti + 1, tc) tiles
+    const int full = tc < tr ? tc : tr;   // rows 0 .. full - 1 hold ti + 1 tiles
+    const int count = LOWER ? full * (full + 1) / 2 + (tr - full) * tc : tr * tc;
     for (int t = wave; t < count; t += 4) {
         int ti, tj;
-        if (LOWER) {          // t = ti (ti + 1) / 2 + tj, tj <= ti
-            ti = 0;
-            while ((ti + 1) * (ti + 2) / 2 <= t) ++ti;
-            tj = t - ti * (ti + 1) / 2;
+        if (LOWER) {
+            if (t < full * (full + 1) / 2) {   // t = ti (ti + 1) / 2 + tj, tj <= ti
+                ti = 0;
+                while ((ti + 1) * (ti + 2) / 2 <= t) ++ti;
+                tj = t - ti * (ti + 1) / 2;
+            } else {
+                const int u = t - full * (full + 1) / 2;
+                ti = full + u / tc;
+                tj = u % tc;
+            }
         } else {
-            ti = t % tb;
-            tj = t / tb;
+            ti = t % tr;
+            tj = t / tr;
         }
         const int r0 = ti * kSmallT, c0 = tj * kSmallT;
-        for (int e = lane; e < kNB * kSmallT; e += 64) {
-            const int i = e % kSmallT, k = e / kSmallT;
-            float va = 0.0f, vb = 0.0f;
-            if (k < kb) {
-                if (r0 + i < m) va = la(r0 + i, k);
-                if (c0 + i < m) vb = rb(k, c0 + i);
-            }
-            sa[k][i] = va;
-            sb[k][i] = vb;
-        }
-        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-        __builtin_amdgcn_wave_barrier();
-        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
         floatx16 acc;
 #pragma unroll
         for (int r = 0; r < 16; ++r) acc[r] = 0.0f;
         const int rc = lane & 31;
-        if (PREC == kPrecFP32) {
+        for (int k0 = 0; k0 < kb; k0 += kNB) {
+            for (int e = lane; e < kNB * kSmallT; e += 64) {
+                const int i = e % kSmallT, k = e / kSmallT;
+                float va = 0.0f, vb = 0.0f;
+                if (k0 + k < kb) {
+                    if (r0 + i < m) va = la(r0 + i, k0 + k);
+                    if (c0 + i < nc) vb = rb(k0 + k, c0 + i);
+                }
+                sa[k][i] = va;
+                sb[k][i] = vb;
+            }
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+            __builtin_amdgcn_wave_barrier();
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+            if (PREC == kPrecFP32) {
 #pragma unroll
-            for (int kk = 0; kk < kNB; kk += 2)
-                acc = __builtin_amdgcn_mfma_f32_32x32x2f32(sb[kk + (lane >> 5)][rc],
-                                                            sa[kk + (lane >> 5)][rc], acc, 0, 0, 0);
-        } else {
+                for (int kk = 0; kk < kNB; kk += 2)
+                    acc = __builtin_amdgcn_mfma_f32_32x32x2f32(sb[kk + (lane >> 5)][rc],
+                                                                sa[kk + (lane >> 5)][rc], acc, 0, 0, 0);
+            } else {
 #pragma unroll
-            for (int kk = 0; kk < kNB; kk += 16) {
-                const int k0 = kk + 8 * (lane >> 5);
-                if (PREC == kPrecBF16) {
-                    bf16x8 a, b;
+                for (int kk = 0; kk < kNB; kk += 16) {
+                    const int k1 = kk + 8 * (lane >> 5);
+                    if (PREC == kPrecBF16) {
+                        bf16x8 a, b;
 #pragma unroll
-                    for (int e = 0; e < 8; ++e) {
-                        a[e] = static_cast<__bf16>(sa[k0 + e][rc]);
-                        b[e] = static_cast<__bf16>(sb[k0 + e][rc]);
+                        for (int e = 0; e < 8; ++e) {
+                            a[e] = static_cast<__bf16>(sa[k1 + e][rc]);
+                            b[e] = static_cast<__bf16>(sb[k1 + e][rc]);
+                        }
+                        acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(b, a, acc, 0, 0, 0);
+                    } else {
+                        halfx8 a, b;
+#pragma unroll
+                        for (int e = 0; e < 8; ++e) {
+                            a[e] = static_cast<_Float16>(sa[k1 + e][rc]);
+                            b[e] = static_cast<_Float16>(sb[k1 + e][rc]);
+                        }
+                        acc = __builtin_amdgcn_mfma_f32_32x32x16_f16(b, a, acc, 0, 0, 0);
                     }
-                    acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(b, a, acc, 0, 0, 0);
-                } else {
-                    halfx8 a, b;
-#pragma unroll
-                    for (int e = 0; e < 8; ++e) {
-                        a[e] = static_cast<_Float16>(sa[k0 + e][rc]);
-                        b[e] = static_cast<_Float16>(sb[k0 + e][rc]);
-                    }
-                    acc = __builtin_amdgcn_mfma_f32_32x32x16_f16(b, a, acc, 0, 0, 0);
                 }
             }
+            // the wave's slice is restaged by its next k slice or tile
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+            __builtin_amdgcn_wave_barrier();
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
         }
         // C^T -= R^T L^T: accumulator r holds C row rc (the lane's),
         // column 8 (r / 4) + 4 (lane / 32) + r % 4
@@ -246,13 +276,9 @@ __device__ void tile_update_small(float* C, long long ldc, int m, int kb, LA la,
 #pragma unroll
             for (int r = 0; r < 16; ++r) {
                 const int gc = c0 + 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3);
-                if (gc < m && (!LOWER || gr >= gc)) C[off(gr, gc, ldc)] -= acc[r];
+                if (gc < nc && (!LOWER || gr >= gc)) C[off(gr, gc, ldc)] -= acc[r];
             }
         }
-        // the wave's slice is restaged by its next tile
-        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-        __builtin_amdgcn_wave_barrier();
-        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
     }
     __syncthreads();
 }
@@ -266,14 +292,17 @@ __device__ inline float lane_value(float v, int lane)
 
 // Right-looking lower Cholesky of one matrix by one workgroup (LAPACK
 // spotrf semantics: info = the first column whose pivot is not positive and
-// finite; the block being factored is written back as far as it got).
-// Per 32-column panel:
+// finite; the block being factored is written back as far as it got), in
+// outer blocks of 64 columns made of two 32-column panels.  Per panel:
 //   1. POTF2 of the diagonal block on wave 0 alone, lane i holding row i in
 //      registers (pivots and column entries broadcast with readlane, no
 //      barrier); L11 and 1 / diag(L11) go to LDS, L11 back to global memory;
 //   2. TRSM A21 := A21 L11^{-T}, one row per thread in registers, L11 read
-//      as workgroup-uniform LDS broadcasts;
-//   3. A22 (lower) -= A21 A21^T on the MFMA tiles.
+//      as workgroup-uniform LDS broadcasts.
+// Between the two panels the second one's 32 columns are updated by the
+// first (a 32-wide strip); after both, the trailing matrix (lower) -=
+// P P^T with the 64-column P on the MFMA tiles, so each element of the
+// trailing matrix is read and written once per 64 columns.
 // A last panel narrower than 32 (kb < 32) reads its column kb - 1 (and, in
 // the POTF2, its row kb - 1) in place of the missing ones: every access
 // stays inside the matrix and nothing computed from the copies is stored.
@@ -286,8 +315,8 @@ __device__ void chol_matrix(float* A, long long lda, int n, int* info, int small
     __shared__ float scratch[kScratch];            // the trailing update's operand buffers
     __shared__ int failed;
     const int tid = threadIdx.x;
-    for (int k = 0; k < n; k += kNB) {
-        const int kb = imin(kNB, n - k);
+    // the panel A(k:n, k:k+kb); false when a pivot fails (info set)
+    auto panel = [&](int k, int kb) -> bool {
         // 1. POTF2
         if (tid < 64) {
             const int lane = tid;
@@ -334,10 +363,8 @@ __device__ void chol_matrix(float* A, long long lda, int n, int* info, int small
             const int i = e % kNB, c = e / kNB;
             if (i < kb && c <= i) A[off(k + i, k + c, lda)] = lt[i][c];
         }
-        if (failed) return;
+        if (failed) return false;
         const int r1 = k + kb;
-        const int m = n - r1;
-        if (m <= 0) break;
         // 2. TRSM
         for (int r = r1 + tid; r < n; r += 256) {
             const float* p = A + off(r, k, lda);
@@ -365,13 +392,36 @@ __device__ void chol_matrix(float* A, long long lda, int n, int* info, int small
             }
         }
         __syncthreads();
-        // 3. SYRK: A22 (lower) -= P P^T with P = A(r1:n, k:k+kb)
+        return true;
+    };
+    // A(r1:n, r1:r1+nc) (lower) -= P P^T with P = A(r1:n, k:k+kw)
+    auto update = [&](int r1, int nc, int k, int kw) {
+        const int m = n - r1;
         float* C = A + off(r1, r1, lda);
         const float* Pp = A + off(r1, k, lda);
         auto la = [&](int i, int kk) { return Pp[off(i, kk, lda)]; };
         auto rb = [&](int kk, int j) { return Pp[off(j, kk, lda)]; };
-        if (m <= small_max) tile_update_small<PREC, true>(C, lda, m, kb, la, rb, scratch);
-        else tile_sweep<PREC, true>(C, lda, m, kb, la, rb, scratch);
+        if (m <= small_max) tile_update_small<PREC, true>(C, lda, m, nc, kw, la, rb, scratch);
+        else tile_sweep<PREC, true, 2>(C, lda, m, nc, kw, la, rb, scratch);
+    };
+    // per outer block: panel a, the strip of panel b by panel a, panel b,
+    // the trailing matrix by both (a runtime loop: one copy of each step's
+    // code)
+    for (int k = 0; k < n; k += 2 * kNB) {
+        const int kb2 = imin(2 * kNB, n - k);
+#pragma nounroll
+        for (int step = 0; step < 4; ++step) {
+            if (step & 1) {
+                const bool strip = step == 1;
+                const int r1 = strip ? k + kNB : k + kb2;
+                if (r1 >= n || (strip && kb2 <= kNB)) continue;
+                update(r1, strip ? kb2 - kNB : n - r1, k, strip ? kNB : kb2);
+            } else {
+                const int kp = k + (step >> 1) * kNB;
+                if (kp >= k + kb2) continue;
+                if (!panel(kp, imin(kNB, k + kb2 - kp))) return;
+            }
+        }
     }
 }
 
@@ -560,11 +610,11 @@ __device__ void lu_matrix(float* A, long long lda, int n, int* ipiv, int* info, 
         auto rb = [&](int kk, int jc) { return U12[off(kk, jc, lda)]; };
         if (mr <= small_max) {
             // the panel array is free once U12 is solved
-            tile_update_small<PREC, false>(C, lda, mr, kb, la, rb, &pnl[0][0]);
+            tile_update_small<PREC, false>(C, lda, mr, mr, kb, la, rb, &pnl[0][0]);
             continue;
         }
         // the panel array is free once U12 is solved
-        tile_sweep<PREC, false>(C, lda, mr, kb, la, rb, &pnl[0][0]);
+        tile_sweep<PREC, false, 1>(C, lda, mr, mr, kb, la, rb, &pnl[0][0]);
     }
 }
 

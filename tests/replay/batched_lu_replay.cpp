@@ -1,7 +1,7 @@
 // Host replay of the batched LU kernel's address arithmetic
 // (hygondoubleblockedfactorizations_amd/csrc/batched.hip: order_kernel,
 // batched_kernel<kPrecFP32, true>, lu_matrix with its panel in LDS,
-// tile_sweep<kPrecFP32, false> with its two operand buffers in the panel
+// tile_sweep<kPrecFP32, false, 1> with its two operand buffers in the panel
 // array and, for trailing orders up to the small size class,
 // tile_update_small<kPrecFP32, false> staging in the panel).
 //
