@@ -428,7 +428,6 @@ __device__ void chol_matrix(float* A, long long lda, int n, int* info, int small
 // ---------------------------------------------------------------- LU
 
 constexpr int kLuMaxOrder = 1024;   // the whole panel (up to 1024 x 32) sits in LDS
-static_assert(kLuMaxOrder >= kNB + 256, "U12 solve scratch rows inside the panel array");
 constexpr int kPanelLd = kNB + 1;   // LDS row stride: conflict-free column walks
 
 // Right-looking LU with partial pivoting of one matrix by one workgroup
@@ -436,8 +435,12 @@ constexpr int kPanelLd = kNB + 1;   // LDS row stride: conflict-free column walk
 // |pivot| >= FLT_MIN, the row interchanges of each panel applied in order to
 // every other column, info = first zero pivot).  Per 32-column panel:
 //   1. the m x kb panel (m = n - k <= 1024 rows) is copied into LDS and
-//      factored there column by column (arg-max, row exchange within the
-//      panel, scaling, rank-1 update of the panel's remaining columns);
+//      factored there column by column, one workgroup barrier per column:
+//      each wave publishes its arg-max candidate for column j with a copy
+//      of the candidate row; after the barrier every thread picks the pivot
+//      row from the four candidates and, for its own rows, applies the
+//      exchange (the owner of the pivot row moves it to row j), the scaling
+//      and the rank-1 update, and tracks its candidate for column j + 1;
 //   2. the factored panel is written back; the panel's interchanges,
 //      composed into one move list (at most 2 x 32 rows), are applied to
 //      the columns left and right of it (one thread per column: every moved
@@ -452,8 +455,9 @@ __device__ void lu_matrix(float* A, long long lda, int n, int* ipiv, int* info, 
     static_assert(kLuMaxOrder * kPanelLd >= kSmallScratch, "small-tile staging inside the panel");
     static_assert(kLuMaxOrder * kPanelLd >= 2 * kTileFloats, "tile buffers inside the panel");
     __shared__ float pnl[kLuMaxOrder][kPanelLd];
-    __shared__ float wv[4];
-    __shared__ int wi[4];
+    __shared__ float cand[2][4][kPanelLd];      // per column parity and wave: the candidate row
+    __shared__ float cand_v[2][4];              // its |value| (-1: no candidate)
+    __shared__ int cand_i[2][4];                // its row
     __shared__ int piv_s[kNB];
     __shared__ int rowmap[kLuMaxOrder];         // panel rows: whose content ends where
     __shared__ int mv_src[2 * kNB], mv_dst[2 * kNB];
@@ -468,18 +472,11 @@ __device__ void lu_matrix(float* A, long long lda, int n, int* ipiv, int* info, 
             pnl[r][c] = (c < kb) ? A[off(k + r, k + c, lda)] : 0.0f;
         }
         __syncthreads();
-        for (int j = 0; j < kb; ++j) {
-            // arg max |pnl(r, j)| over r >= j: first index among equal
-            // magnitudes, NaN never wins
-            float bv = -1.0f;
-            int bi = m;
-            for (int r = j + tid; r < m; r += 256) {
-                const float v = fabsf(pnl[r][j]);
-                if (v > bv) {
-                    bv = v;
-                    bi = r;
-                }
-            }
+        // wave arg max of (bv, bi) (first index among equal magnitudes; NaN
+        // never wins), published with a copy of the winning row; rows
+        // written by this wave's lanes in the same pass are seen (a wave's
+        // LDS accesses complete in order)
+        auto publish = [&](int par, float bv, int bi) {
 #pragma unroll
             for (int o = 32; o > 0; o >>= 1) {
                 const float ov = __shfl_xor(bv, o, 64);
@@ -490,42 +487,78 @@ __device__ void lu_matrix(float* A, long long lda, int n, int* ipiv, int* info, 
                 }
             }
             if (lane == 0) {
-                wv[wave] = bv;
-                wi[wave] = bi;
+                cand_v[par][wave] = bv;
+                cand_i[par][wave] = bi;
             }
-            __syncthreads();
-            float pv = wv[0];
-            int p = wi[0];
-#pragma unroll
-            for (int w = 1; w < 4; ++w)
-                if (wv[w] > pv || (wv[w] == pv && wi[w] < p)) {
-                    pv = wv[w];
-                    p = wi[w];
+            if (lane < kNB) cand[par][wave][lane] = bi < m ? pnl[bi][lane] : 0.0f;
+        };
+        {
+            float bv = -1.0f;
+            int bi = m;
+            for (int r = tid; r < m; r += 256) {
+                const float v = fabsf(pnl[r][0]);
+                if (v > bv) {
+                    bv = v;
+                    bi = r;
                 }
-            if (p < j || p >= m) p = j;   // no candidate (all NaN): keep the diagonal
-            const float piv = pnl[p][j];
+            }
+            publish(0, bv, bi);
+        }
+        __syncthreads();
+        for (int j = 0; j < kb; ++j) {
+            const int par = j & 1;
+            float pv = cand_v[par][0];
+            int p = cand_i[par][0], w = 0;
+#pragma unroll
+            for (int q = 1; q < 4; ++q)
+                if (cand_v[par][q] > pv || (cand_v[par][q] == pv && cand_i[par][q] < p)) {
+                    pv = cand_v[par][q];
+                    p = cand_i[par][q];
+                    w = q;
+                }
+            // no candidate (all NaN): keep the diagonal
+            const bool none = p < j || p >= m;
+            if (none) p = j;
+            const float* prow = none ? &pnl[j][0] : &cand[par][w][0];
+            const float piv = prow[j];
             if (tid == 0) {
                 ipiv[k + j] = k + p + 1;
                 piv_s[j] = p;
                 if (piv == 0.0f && *info == 0) *info = k + j + 1;
             }
-            __syncthreads();   // every thread has read the pivot before the rows move
-            if (p != j && tid < kNB) {
-                const float t = pnl[j][tid];
-                pnl[j][tid] = pnl[p][tid];
-                pnl[p][tid] = t;
-            }
-            __syncthreads();
-            if (piv != 0.0f) {
-                const bool recip = fabsf(piv) >= sfmin;
-                const float rinv = 1.0f / piv;
-                for (int r = j + 1 + tid; r < m; r += 256) {
-                    const float x = pnl[r][j];
+            const bool scale = piv != 0.0f;
+            const bool recip = fabsf(piv) >= sfmin;
+            const float rinv = 1.0f / piv;
+            float bv = -1.0f;
+            int bi = m;
+            for (int r = tid; r < m; r += 256) {
+                if (r <= j) continue;
+                float* dst = &pnl[r][0];
+                if (r == p) {
+                    // the pivot row moves to row j, row j's content to row p
+                    float old[kNB];
+#pragma unroll
+                    for (int c = 0; c < kNB; ++c) old[c] = pnl[j][c];
+#pragma unroll
+                    for (int c = 0; c < kNB; ++c) pnl[j][c] = prow[c];
+#pragma unroll
+                    for (int c = 0; c < kNB; ++c) dst[c] = old[c];
+                }
+                if (scale) {
+                    const float x = dst[j];
                     const float l = recip ? x * rinv : x / piv;
-                    pnl[r][j] = l;
-                    for (int c = j + 1; c < kb; ++c) pnl[r][c] -= l * pnl[j][c];
+                    dst[j] = l;
+                    for (int c = j + 1; c < kb; ++c) dst[c] -= l * prow[c];
+                }
+                if (j + 1 < kb) {
+                    const float v = fabsf(dst[j + 1]);
+                    if (v > bv) {
+                        bv = v;
+                        bi = r;
+                    }
                 }
             }
+            if (j + 1 < kb) publish(par ^ 1, bv, bi);
             __syncthreads();
         }
         // 2. panel -> global; the panel's interchanges to the other columns
@@ -585,21 +618,26 @@ __device__ void lu_matrix(float* A, long long lda, int n, int* ipiv, int* info, 
         const int c1 = k + kb;
         const int mr = n - c1;
         if (mr <= 0) continue;
-        // 3. U12 := L11^{-1} A12 (unit lower L11 = pnl rows 0..kb), one thread
-        //    per column; the column sits in LDS row kNB + tid of the panel
-        //    array (free once the panel is written back)
-        for (int c0 = c1; c0 < n; c0 += 256) {
-            const int c = c0 + tid;
-            float* x = pnl[kNB + tid];
-            if (c < n) {
-                for (int i = 0; i < kb; ++i) x[i] = A[off(k + i, c, lda)];
-                for (int i = 1; i < kb; ++i) {
-                    float v = x[i];
-                    for (int q = 0; q < i; ++q) v -= pnl[i][q] * x[q];
-                    x[i] = v;
-                }
-                for (int i = 0; i < kb; ++i) A[off(k + i, c, lda)] = x[i];
+        // 3. U12 := L11^{-1} A12 (unit lower L11 = pnl rows 0..kb, kb = 32
+        //    here: a narrower panel is the last one), one column per thread
+        //    in registers, L11 read as workgroup-uniform LDS broadcasts
+        for (int c = c1 + tid; c < n; c += 256) {
+            float* p = A + off(k, c, lda);
+            float x[kNB];
+#pragma unroll
+            for (int i = 0; i < kNB; ++i) x[i] = p[i];
+#pragma unroll
+            for (int i = 1; i < kNB; ++i) {
+                float v = x[i];
+#pragma unroll
+                for (int q = 0; q < i; ++q) v -= pnl[i][q] * x[q];
+                x[i] = v;
+                // row i + 1 of L11 is read after x[i] is done (all 496
+                // entries ahead of the arithmetic would spill)
+                asm volatile("" : "+v"(x[i]) : : "memory");
             }
+#pragma unroll
+            for (int i = 0; i < kNB; ++i) p[i] = x[i];
         }
         __syncthreads();
         // 4. A22 -= L21 U12

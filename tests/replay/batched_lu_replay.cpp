@@ -182,6 +182,10 @@ void lu_matrix(Checked& M, int n, int* info)
     static Lds<int, 1, kLuMaxOrder> rowmap;
     Lds<int, 1, 2 * kNB> mv_src, mv_dst;
     rowmap.violations = mv_src.violations = mv_dst.violations = &M.violations;
+    Lds<float, 8, kNB + 1> cand;
+    Lds<float, 2, 4> cand_v;
+    Lds<int, 2, 4> cand_i;
+    cand.violations = cand_v.violations = cand_i.violations = &M.violations;
     pnl.violations = piv_s.violations = &M.violations;
     for (int k = 0; k < n; k += kNB) {
         const int kb = std::min(kNB, n - k);
@@ -192,22 +196,9 @@ void lu_matrix(Checked& M, int n, int* info)
                 const int r = e % m, c = e / m;
                 pnl.at(r, c) = (c < kb) ? M.A(k + r, k + c) : 0.0f;
             }
-        for (int j = 0; j < kb; ++j) {
-            float bv[kThreads];
-            int bi[kThreads];
-            for (int tid = 0; tid < kThreads; ++tid) {
-                bv[tid] = -1.0f;
-                bi[tid] = m;
-                for (int r = j + tid; r < m; r += kThreads) {
-                    const float v = std::fabs(pnl.at(r, j));
-                    if (v > bv[tid]) {
-                        bv[tid] = v;
-                        bi[tid] = r;
-                    }
-                }
-            }
-            float wv[4];
-            int wi[4];
+        // each wave's arg-max candidate (butterfly lane by lane) and a copy
+        // of its row, per column parity
+        auto publish = [&](int par, std::vector<float>& bv, std::vector<int>& bi) {
             for (int w = 0; w < 4; ++w) {
                 for (int o = 32; o > 0; o >>= 1) {
                     float nv[64];
@@ -229,33 +220,71 @@ void lu_matrix(Checked& M, int n, int* info)
                         bi[w * 64 + l] = ni[l];
                     }
                 }
-                wv[w] = bv[w * 64];
-                wi[w] = bi[w * 64];
+                cand_v.at(par, w) = bv[w * 64];
+                cand_i.at(par, w) = bi[w * 64];
+                for (int l = 0; l < kNB; ++l)
+                    cand.at(par * 4 + w, l) = bi[w * 64] < m ? pnl.at(bi[w * 64], l) : 0.0f;
             }
-            float pv = wv[0];
-            int p = wi[0];
-            for (int w = 1; w < 4; ++w)
-                if (wv[w] > pv || (wv[w] == pv && wi[w] < p)) {
-                    pv = wv[w];
-                    p = wi[w];
+        };
+        {
+            std::vector<float> bv(kThreads, -1.0f);
+            std::vector<int> bi(kThreads, m);
+            for (int tid = 0; tid < kThreads; ++tid)
+                for (int r = tid; r < m; r += kThreads) {
+                    const float v = std::fabs(pnl.at(r, 0));
+                    if (v > bv[tid]) {
+                        bv[tid] = v;
+                        bi[tid] = r;
+                    }
                 }
-            if (p < j || p >= m) p = j;
-            const float piv = pnl.at(p, j);
+            publish(0, bv, bi);
+        }
+        for (int j = 0; j < kb; ++j) {
+            const int par = j & 1;
+            float pv = cand_v.at(par, 0);
+            int p = cand_i.at(par, 0), w = 0;
+            for (int q = 1; q < 4; ++q)
+                if (cand_v.at(par, q) > pv || (cand_v.at(par, q) == pv && cand_i.at(par, q) < p)) {
+                    pv = cand_v.at(par, q);
+                    p = cand_i.at(par, q);
+                    w = q;
+                }
+            const bool none = p < j || p >= m;
+            if (none) p = j;
+            auto prow = [&](int c) { return none ? pnl.at(j, c) : cand.at(par * 4 + w, c); };
+            const float piv = prow(j);
             M.piv(k + j) = k + p + 1;
             piv_s.at(0, j) = p;
             if (piv == 0.0f && *info == 0) *info = k + j + 1;
-            if (p != j)
-                for (int t = 0; t < kNB; ++t) std::swap(pnl.at(j, t), pnl.at(p, t));
-            if (piv != 0.0f) {
-                const bool recip = std::fabs(piv) >= FLT_MIN;
-                const float rinv = 1.0f / piv;
-                for (int r = j + 1; r < m; ++r) {
-                    const float x = pnl.at(r, j);
-                    const float l = recip ? x * rinv : x / piv;
-                    pnl.at(r, j) = l;
-                    for (int c = j + 1; c < kb; ++c) pnl.at(r, c) -= l * pnl.at(j, c);
+            const bool scale = piv != 0.0f;
+            const bool recip = std::fabs(piv) >= FLT_MIN;
+            const float rinv = 1.0f / piv;
+            std::vector<float> bv(kThreads, -1.0f);
+            std::vector<int> bi(kThreads, m);
+            for (int tid = 0; tid < kThreads; ++tid)
+                for (int r = tid; r < m; r += kThreads) {
+                    if (r <= j) continue;
+                    if (r == p) {
+                        float old[kNB];
+                        for (int c = 0; c < kNB; ++c) old[c] = pnl.at(j, c);
+                        for (int c = 0; c < kNB; ++c) pnl.at(j, c) = prow(c);
+                        for (int c = 0; c < kNB; ++c) pnl.at(r, c) = old[c];
+                    }
+                    if (scale) {
+                        const float x = pnl.at(r, j);
+                        const float l = recip ? x * rinv : x / piv;
+                        pnl.at(r, j) = l;
+                        for (int c = j + 1; c < kb; ++c) pnl.at(r, c) -= l * prow(c);
+                    }
+                    if (j + 1 < kb) {
+                        const float v = std::fabs(pnl.at(r, j + 1));
+                        if (v > bv[tid]) {
+                            bv[tid] = v;
+                            bi[tid] = r;
+                        }
+                    }
                 }
-            }
+            if (j + 1 < kb) publish(par ^ 1, bv, bi);
         }
         // 2. panel -> global; interchanges to the other columns
         for (int e = 0; e < m * kb; ++e) {
@@ -298,19 +327,19 @@ void lu_matrix(Checked& M, int n, int* info)
         }
         const int c1 = k + kb, mr = n - c1;
         if (mr <= 0) continue;
-        // 3. U12 := L11^{-1} A12, thread tid's column in LDS row kNB + tid
-        for (int c0 = c1; c0 < n; c0 += kThreads)
-            for (int tid = 0; tid < kThreads; ++tid) {
-                const int c = c0 + tid;
-                if (c >= n) continue;
-                for (int ii = 0; ii < kb; ++ii) pnl.at(kNB + tid, ii) = M.A(k + ii, c);
-                for (int ii = 1; ii < kb; ++ii) {
-                    float v = pnl.at(kNB + tid, ii);
-                    for (int q = 0; q < ii; ++q) v -= pnl.at(ii, q) * pnl.at(kNB + tid, q);
-                    pnl.at(kNB + tid, ii) = v;
-                }
-                for (int ii = 0; ii < kb; ++ii) M.A(k + ii, c) = pnl.at(kNB + tid, ii);
+        // 3. U12 := L11^{-1} A12, one column per thread in registers (the
+        //    kernel reads all 32 rows: a panel narrower than 32 is the last)
+        if (kb != kNB) M.report("U12 with a narrow panel", k, kb);
+        for (int c = c1; c < n; ++c) {
+            float x[kNB];
+            for (int ii = 0; ii < kNB; ++ii) x[ii] = M.A(k + ii, c);
+            for (int ii = 1; ii < kNB; ++ii) {
+                float v = x[ii];
+                for (int q = 0; q < ii; ++q) v -= pnl.at(ii, q) * x[q];
+                x[ii] = v;
             }
+            for (int ii = 0; ii < kNB; ++ii) M.A(k + ii, c) = x[ii];
+        }
         // 4. A22 -= L21 U12
         if (mr <= small_max) {
             tile_update_small(M, pnl, c1, k, mr, kb);
