@@ -438,9 +438,10 @@ constexpr int kPanelLd = kNB + 1;   // LDS row stride: conflict-free column walk
 //      factored there column by column, one workgroup barrier per column:
 //      each wave publishes its arg-max candidate for column j with a copy
 //      of the candidate row; after the barrier every thread picks the pivot
-//      row from the four candidates and, for its own rows, applies the
-//      exchange (the owner of the pivot row moves it to row j), the scaling
-//      and the rank-1 update, and tracks its candidate for column j + 1;
+//      row from the four candidates; the wave owning the pivot row
+//      exchanges it with row j, and every thread applies the scaling and
+//      the rank-1 update to its own rows and tracks its candidate for
+//      column j + 1;
 //   2. the factored panel is written back; the panel's interchanges,
 //      composed into one move list (at most 2 x 32 rows), are applied to
 //      the columns left and right of it (one thread per column: every moved
@@ -466,10 +467,19 @@ __device__ void lu_matrix(float* A, long long lda, int n, int* ipiv, int* info, 
     for (int k = 0; k < n; k += kNB) {
         const int kb = imin(kNB, n - k);
         const int m = n - k;
-        // 1. panel -> LDS (coalesced down each column)
-        for (int e = tid; e < m * kNB; e += 256) {
-            const int r = e % m, c = e / m;
-            pnl[r][c] = (c < kb) ? A[off(k + r, k + c, lda)] : 0.0f;
+        // 1. panel -> LDS: thread tid takes rows tid, tid + 256, ..., each
+        //    row's 32 loads in flight together (coalesced across the wave:
+        //    consecutive rows of one column); columns past kb are zero
+        for (int r = tid; r < m; r += 256) {
+            const float* p = A + off(k + r, k, lda);
+            float v[kNB];
+#pragma unroll
+            for (int c = 0; c < kNB; ++c) {
+                v[c] = c < kb ? *p : 0.0f;
+                if (c + 1 < kb) p += lda;
+            }
+#pragma unroll
+            for (int c = 0; c < kNB; ++c) pnl[r][c] = v[c];
         }
         __syncthreads();
         // wave arg max of (bv, bi) (first index among equal magnitudes; NaN
@@ -529,21 +539,19 @@ __device__ void lu_matrix(float* A, long long lda, int n, int* ipiv, int* info, 
             const bool scale = piv != 0.0f;
             const bool recip = fabsf(piv) >= sfmin;
             const float rinv = 1.0f / piv;
+            // the exchange, by the wave that owns row p (its lane p % 64
+            // updates row p below, after these stores: a wave's LDS
+            // accesses complete in order); no other wave touches rows j, p
+            if (p != j && wave == ((p & 255) >> 6) && lane < kNB) {
+                const float t = pnl[j][lane];
+                pnl[j][lane] = prow[lane];
+                pnl[p][lane] = t;
+            }
             float bv = -1.0f;
             int bi = m;
             for (int r = tid; r < m; r += 256) {
                 if (r <= j) continue;
                 float* dst = &pnl[r][0];
-                if (r == p) {
-                    // the pivot row moves to row j, row j's content to row p
-                    float old[kNB];
-#pragma unroll
-                    for (int c = 0; c < kNB; ++c) old[c] = pnl[j][c];
-#pragma unroll
-                    for (int c = 0; c < kNB; ++c) pnl[j][c] = prow[c];
-#pragma unroll
-                    for (int c = 0; c < kNB; ++c) dst[c] = old[c];
-                }
                 if (scale) {
                     const float x = dst[j];
                     const float l = recip ? x * rinv : x / piv;
@@ -561,10 +569,18 @@ __device__ void lu_matrix(float* A, long long lda, int n, int* ipiv, int* info, 
             if (j + 1 < kb) publish(par ^ 1, bv, bi);
             __syncthreads();
         }
-        // 2. panel -> global; the panel's interchanges to the other columns
-        for (int e = tid; e < m * kb; e += 256) {
-            const int r = e % m, c = e / m;
-            A[off(k + r, k + c, lda)] = pnl[r][c];
+        // 2. panel -> global (the same row split); the panel's interchanges
+        //    to the other columns
+        for (int r = tid; r < m; r += 256) {
+            float* p = A + off(k + r, k, lda);
+            float v[kNB];
+#pragma unroll
+            for (int c = 0; c < kNB; ++c) v[c] = pnl[r][c];
+#pragma unroll
+            for (int c = 0; c < kNB; ++c) {
+                if (c < kb) *p = v[c];
+                if (c + 1 < kb) p += lda;
+            }
         }
         // the panel's interchanges, composed: rowmap[r] = the row whose
         // content ends at row r, for r in {0..kb-1} and the pivot rows

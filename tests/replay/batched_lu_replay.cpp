@@ -190,12 +190,10 @@ void lu_matrix(Checked& M, int n, int* info)
     for (int k = 0; k < n; k += kNB) {
         const int kb = std::min(kNB, n - k);
         const int m = n - k;
-        // 1. panel -> LDS
+        // 1. panel -> LDS, rows tid, tid + 256, ... per thread
         for (int tid = 0; tid < kThreads; ++tid)
-            for (int e = tid; e < m * kNB; e += kThreads) {
-                const int r = e % m, c = e / m;
-                pnl.at(r, c) = (c < kb) ? M.A(k + r, k + c) : 0.0f;
-            }
+            for (int r = tid; r < m; r += kThreads)
+                for (int c = 0; c < kNB; ++c) pnl.at(r, c) = c < kb ? M.A(k + r, k + c) : 0.0f;
         // each wave's arg-max candidate (butterfly lane by lane) and a copy
         // of its row, per column parity
         auto publish = [&](int par, std::vector<float>& bv, std::vector<int>& bi) {
@@ -259,17 +257,19 @@ void lu_matrix(Checked& M, int n, int* info)
             const bool scale = piv != 0.0f;
             const bool recip = std::fabs(piv) >= FLT_MIN;
             const float rinv = 1.0f / piv;
+            // the exchange by the wave owning row p (lanes 0..31), before its
+            // lane p % 64 updates row p; no other wave touches rows j, p
+            if (p != j)
+                for (int l = 0; l < kNB; ++l) {
+                    const float t = pnl.at(j, l);
+                    pnl.at(j, l) = prow(l);
+                    pnl.at(p, l) = t;
+                }
             std::vector<float> bv(kThreads, -1.0f);
             std::vector<int> bi(kThreads, m);
             for (int tid = 0; tid < kThreads; ++tid)
                 for (int r = tid; r < m; r += kThreads) {
                     if (r <= j) continue;
-                    if (r == p) {
-                        float old[kNB];
-                        for (int c = 0; c < kNB; ++c) old[c] = pnl.at(j, c);
-                        for (int c = 0; c < kNB; ++c) pnl.at(j, c) = prow(c);
-                        for (int c = 0; c < kNB; ++c) pnl.at(r, c) = old[c];
-                    }
                     if (scale) {
                         const float x = pnl.at(r, j);
                         const float l = recip ? x * rinv : x / piv;
@@ -287,10 +287,9 @@ void lu_matrix(Checked& M, int n, int* info)
             if (j + 1 < kb) publish(par ^ 1, bv, bi);
         }
         // 2. panel -> global; interchanges to the other columns
-        for (int e = 0; e < m * kb; ++e) {
-            const int r = e % m, c = e / m;
-            M.A(k + r, k + c) = pnl.at(r, c);
-        }
+        for (int tid = 0; tid < kThreads; ++tid)
+            for (int r = tid; r < m; r += kThreads)
+                for (int c = 0; c < kb; ++c) M.A(k + r, k + c) = pnl.at(r, c);
         // the panel's interchanges composed into a move list (rowmap: whose
         // content ends at each panel row), then every other column: all
         // moved rows read, then written
