@@ -442,12 +442,14 @@ constexpr int kPanelLd = kNB + 1;   // LDS row stride: conflict-free column walk
 //      exchanges it with row j, and every thread applies the scaling and
 //      the rank-1 update to its own rows and tracks its candidate for
 //      column j + 1;
-//   2. the factored panel is written back; the panel's interchanges,
-//      composed into one move list (at most 2 x 32 rows), are applied to
-//      the columns left and right of it (one thread per column: every moved
-//      row read, then written);
-//   3. U12 := L11^{-1} A12 (unit lower, L11 from LDS; one thread per column);
+//   2. the factored panel is written back; its interchanges are composed
+//      into one move list (at most 2 x 32 rows);
+//   3. each column right of the panel (one per thread, in registers): the
+//      moves, then U12 := L11^{-1} A12 (unit lower, L11 from LDS);
 //   4. A22 -= L21 U12 on MFMA tiles.
+// The columns left of a panel are not read again, so the interchanges of
+// the later panels reach them once, at the end: per panel, its rows below
+// the panel staged in LDS and written to their final rows.
 // Every global access is to (row, column) of the n x n matrix with both
 // below n (the caller checks lda >= n), every LDS access below its bounds.
 template <int PREC>
@@ -461,6 +463,7 @@ __device__ void lu_matrix(float* A, long long lda, int n, int* ipiv, int* info, 
     __shared__ int cand_i[2][4];                // its row
     __shared__ int piv_s[kNB];
     __shared__ int rowmap[kLuMaxOrder];         // panel rows: whose content ends where
+    __shared__ int piv_all[kLuMaxOrder];        // every column's pivot row (absolute)
     __shared__ int mv_src[2 * kNB], mv_dst[2 * kNB];
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     constexpr float sfmin = FLT_MIN;
@@ -534,6 +537,7 @@ __device__ void lu_matrix(float* A, long long lda, int n, int* ipiv, int* info, 
             if (tid == 0) {
                 ipiv[k + j] = k + p + 1;
                 piv_s[j] = p;
+                piv_all[k + j] = k + p;
                 if (piv == 0.0f && *info == 0) *info = k + j + 1;
             }
             const bool scale = piv != 0.0f;
@@ -582,6 +586,9 @@ __device__ void lu_matrix(float* A, long long lda, int n, int* ipiv, int* info, 
                 if (c + 1 < kb) p += lda;
             }
         }
+        const int c1 = k + kb;
+        const int mr = n - c1;
+        if (mr <= 0) continue;   // the last panel (kb = 32 below)
         // the panel's interchanges, composed: rowmap[r] = the row whose
         // content ends at row r, for r in {0..kb-1} and the pivot rows
         if (tid < kb) {
@@ -601,7 +608,8 @@ __device__ void lu_matrix(float* A, long long lda, int n, int* ipiv, int* info, 
         }
         __syncthreads();
         // move list: entry t < kb for row t, entry kb + u for pivot row
-        // piv_s[u] below the panel's first kb rows (its first occurrence);
+        // piv_s[u] below the panel's first kb rows (its first occurrence;
+        // its new content is one of the first kb rows' old contents);
         // src -1 where nothing moves
         if (tid < 2 * kNB) {
             int dst = -1;
@@ -619,29 +627,21 @@ __device__ void lu_matrix(float* A, long long lda, int n, int* ipiv, int* info, 
             mv_src[tid] = (dst >= 0 && src != dst) ? src : -1;
         }
         __syncthreads();
-        // every other column: all moved rows read, then written
-        for (int t = tid; t < n - kb; t += 256) {
-            const int c = t < k ? t : t + kb;
-            float v[2 * kNB];
-#pragma unroll
-            for (int i = 0; i < 2 * kNB; ++i)
-                v[i] = mv_src[i] >= 0 ? A[off(k + mv_src[i], c, lda)] : 0.0f;
-#pragma unroll
-            for (int i = 0; i < 2 * kNB; ++i)
-                if (mv_src[i] >= 0) A[off(k + mv_dst[i], c, lda)] = v[i];
-        }
-        __syncthreads();
-        const int c1 = k + kb;
-        const int mr = n - c1;
-        if (mr <= 0) continue;
-        // 3. U12 := L11^{-1} A12 (unit lower L11 = pnl rows 0..kb, kb = 32
-        //    here: a narrower panel is the last one), one column per thread
-        //    in registers, L11 read as workgroup-uniform LDS broadcasts
+        // 3. the columns right of the panel, one per thread in registers:
+        //    the interchanges (rows 0..31 read from their sources, the pivot
+        //    rows below written from theirs: every read before any write),
+        //    then U12 := L11^{-1} A12 (unit lower L11 = pnl rows 0..31,
+        //    read as workgroup-uniform LDS broadcasts)
         for (int c = c1 + tid; c < n; c += 256) {
             float* p = A + off(k, c, lda);
-            float x[kNB];
+            float x[kNB], y[kNB];
 #pragma unroll
-            for (int i = 0; i < kNB; ++i) x[i] = p[i];
+            for (int i = 0; i < kNB; ++i) x[i] = p[mv_src[i] >= 0 ? mv_src[i] : i];
+#pragma unroll
+            for (int u = 0; u < kNB; ++u) y[u] = mv_src[kNB + u] >= 0 ? p[mv_src[kNB + u]] : 0.0f;
+#pragma unroll
+            for (int u = 0; u < kNB; ++u)
+                if (mv_src[kNB + u] >= 0) p[mv_dst[kNB + u]] = y[u];
 #pragma unroll
             for (int i = 1; i < kNB; ++i) {
                 float v = x[i];
@@ -669,6 +669,54 @@ __device__ void lu_matrix(float* A, long long lda, int n, int* ipiv, int* info, 
         }
         // the panel array is free once U12 is solved
         tile_sweep<PREC, false, 1>(C, lda, mr, mr, kb, la, rb, &pnl[0][0]);
+    }
+    // 5. the interchanges of the later panels, applied to the columns of
+    //    each earlier panel once at the end (no step reads those columns
+    //    after their panel): inv = the composition of the later panels'
+    //    exchanges (rowmap), built backwards; rows [kn, n) of the panel's
+    //    columns staged in LDS, then each row written to inv(row)
+    const int panels = (n + kNB - 1) / kNB;
+    if (panels > 1) {
+        for (int r = tid; r < n; r += 256) rowmap[r] = r;
+        __syncthreads();
+        for (int b = panels - 2; b >= 0; --b) {
+            const int kn = (b + 1) * kNB;
+            if (tid == 0) {
+                for (int t = imin(kNB, n - kn) - 1; t >= 0; --t) {
+                    const int j = kn + t, p = piv_all[j];
+                    const int v = rowmap[j];
+                    rowmap[j] = rowmap[p];
+                    rowmap[p] = v;
+                }
+            }
+            __syncthreads();
+            for (int r = kn + tid; r < n; r += 256) {
+                const float* q = A + off(r, b * kNB, lda);
+                float v[kNB];
+#pragma unroll
+                for (int c = 0; c < kNB; ++c) {
+                    v[c] = *q;
+                    if (c + 1 < kNB) q += lda;
+                }
+#pragma unroll
+                for (int c = 0; c < kNB; ++c) pnl[r - kn][c] = v[c];
+            }
+            __syncthreads();
+            for (int r = kn + tid; r < n; r += 256) {
+                const int d = rowmap[r];
+                if (d == r) continue;
+                float* q = A + off(d, b * kNB, lda);
+                float v[kNB];
+#pragma unroll
+                for (int c = 0; c < kNB; ++c) v[c] = pnl[r - kn][c];
+#pragma unroll
+                for (int c = 0; c < kNB; ++c) {
+                    *q = v[c];
+                    if (c + 1 < kNB) q += lda;
+                }
+            }
+            __syncthreads();
+        }
     }
 }
 

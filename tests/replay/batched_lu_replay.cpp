@@ -182,6 +182,8 @@ void lu_matrix(Checked& M, int n, int* info)
     static Lds<int, 1, kLuMaxOrder> rowmap;
     Lds<int, 1, 2 * kNB> mv_src, mv_dst;
     rowmap.violations = mv_src.violations = mv_dst.violations = &M.violations;
+    static Lds<int, 1, kLuMaxOrder> piv_all;
+    piv_all.violations = &M.violations;
     Lds<float, 8, kNB + 1> cand;
     Lds<float, 2, 4> cand_v;
     Lds<int, 2, 4> cand_i;
@@ -253,6 +255,7 @@ void lu_matrix(Checked& M, int n, int* info)
             const float piv = prow(j);
             M.piv(k + j) = k + p + 1;
             piv_s.at(0, j) = p;
+            piv_all.at(0, k + j) = k + p;
             if (piv == 0.0f && *info == 0) *info = k + j + 1;
             const bool scale = piv != 0.0f;
             const bool recip = std::fabs(piv) >= FLT_MIN;
@@ -290,9 +293,11 @@ void lu_matrix(Checked& M, int n, int* info)
         for (int tid = 0; tid < kThreads; ++tid)
             for (int r = tid; r < m; r += kThreads)
                 for (int c = 0; c < kb; ++c) M.A(k + r, k + c) = pnl.at(r, c);
+        const int c1 = k + kb, mr = n - c1;
+        if (mr <= 0) continue;
         // the panel's interchanges composed into a move list (rowmap: whose
-        // content ends at each panel row), then every other column: all
-        // moved rows read, then written
+        // content ends at each panel row)
+        if (kb != kNB) M.report("right columns after a narrow panel", k, kb);
         for (int tid = 0; tid < kb; ++tid) {
             rowmap.at(0, tid) = tid;
             rowmap.at(0, piv_s.at(0, tid)) = piv_s.at(0, tid);
@@ -316,22 +321,17 @@ void lu_matrix(Checked& M, int n, int* info)
             mv_dst.at(0, tid) = dst;
             mv_src.at(0, tid) = (dst >= 0 && src != dst) ? src : -1;
         }
-        for (int t = 0; t < n - kb; ++t) {
-            const int c = t < k ? t : t + kb;
-            float v[2 * kNB];
-            for (int i = 0; i < 2 * kNB; ++i)
-                v[i] = mv_src.at(0, i) >= 0 ? M.A(k + mv_src.at(0, i), c) : 0.0f;
-            for (int i = 0; i < 2 * kNB; ++i)
-                if (mv_src.at(0, i) >= 0) M.A(k + mv_dst.at(0, i), c) = v[i];
-        }
-        const int c1 = k + kb, mr = n - c1;
-        if (mr <= 0) continue;
-        // 3. U12 := L11^{-1} A12, one column per thread in registers (the
-        //    kernel reads all 32 rows: a panel narrower than 32 is the last)
-        if (kb != kNB) M.report("U12 with a narrow panel", k, kb);
+        // 3. each column right of the panel: rows 0..31 read from their
+        //    sources, the pivot rows below written from theirs (every read
+        //    before any write), then U12 := L11^{-1} A12 in registers
         for (int c = c1; c < n; ++c) {
-            float x[kNB];
-            for (int ii = 0; ii < kNB; ++ii) x[ii] = M.A(k + ii, c);
+            float x[kNB], y[kNB];
+            for (int ii = 0; ii < kNB; ++ii)
+                x[ii] = M.A(k + (mv_src.at(0, ii) >= 0 ? mv_src.at(0, ii) : ii), c);
+            for (int u = 0; u < kNB; ++u)
+                y[u] = mv_src.at(0, kNB + u) >= 0 ? M.A(k + mv_src.at(0, kNB + u), c) : 0.0f;
+            for (int u = 0; u < kNB; ++u)
+                if (mv_src.at(0, kNB + u) >= 0) M.A(k + mv_dst.at(0, kNB + u), c) = y[u];
             for (int ii = 1; ii < kNB; ++ii) {
                 float v = x[ii];
                 for (int q = 0; q < ii; ++q) v -= pnl.at(ii, q) * x[q];
@@ -345,6 +345,27 @@ void lu_matrix(Checked& M, int n, int* info)
             continue;
         }
         tile_sweep(M, pnl, c1, k, mr, kb);
+    }
+    // 5. the later panels' interchanges applied to each earlier panel's
+    //    columns at the end: inv (rowmap) composed backwards, rows [kn, n)
+    //    staged in the panel array, each written to inv(row)
+    const int panels = (n + kNB - 1) / kNB;
+    if (panels > 1) {
+        for (int r = 0; r < n; ++r) rowmap.at(0, r) = r;
+        for (int b = panels - 2; b >= 0; --b) {
+            const int kn = (b + 1) * kNB;
+            for (int t = std::min(kNB, n - kn) - 1; t >= 0; --t) {
+                const int j = kn + t, pp = piv_all.at(0, j);
+                std::swap(rowmap.at(0, j), rowmap.at(0, pp));
+            }
+            for (int r = kn; r < n; ++r)
+                for (int c = 0; c < kNB; ++c) pnl.at(r - kn, c) = M.A(r, b * kNB + c);
+            for (int r = kn; r < n; ++r) {
+                const int d = rowmap.at(0, r);
+                if (d == r) continue;
+                for (int c = 0; c < kNB; ++c) M.A(d, b * kNB + c) = pnl.at(r - kn, c);
+            }
+        }
     }
 }
 
