@@ -433,7 +433,8 @@ constexpr int kPanelLd = kNB + 1;   // LDS row stride: conflict-free column walk
 // Right-looking LU with partial pivoting of one matrix by one workgroup
 // (LAPACK sgetrf semantics: IxAMAX first of ties, reciprocal scaling when
 // |pivot| >= FLT_MIN, the row interchanges of each panel applied in order to
-// every other column, info = first zero pivot).  Per 32-column panel:
+// every other column, info = first zero pivot), in outer blocks of 64
+// columns made of two 32-column panels.  Per panel:
 //   1. the m x kb panel (m = n - k <= 1024 rows) is copied into LDS and
 //      factored there column by column, one workgroup barrier per column:
 //      each wave publishes its arg-max candidate for column j with a copy
@@ -443,13 +444,18 @@ constexpr int kPanelLd = kNB + 1;   // LDS row stride: conflict-free column walk
 //      the rank-1 update to its own rows and tracks its candidate for
 //      column j + 1;
 //   2. the factored panel is written back; its interchanges are composed
-//      into one move list (at most 2 x 32 rows);
+//      into one move list (at most 2 x 32 rows); the second panel of a
+//      block applies its moves to the first one's columns too;
 //   3. each column right of the panel (one per thread, in registers): the
-//      moves, then U12 := L11^{-1} A12 (unit lower, L11 from LDS);
-//   4. A22 -= L21 U12 on MFMA tiles.
-// The columns left of a panel are not read again, so the interchanges of
-// the later panels reach them once, at the end: per panel, its rows below
-// the panel staged in LDS and written to their final rows.
+//      moves, (second panel) minus the first panel's part, L_ba U_a, then
+//      U := L11^{-1} (.) (unit lower, L11 from LDS);
+//   4. first panel: the second panel's columns -= L21 U12 (a 32-wide strip
+//      on the MFMA tiles); second panel: the trailing matrix -= the two
+//      panels' L21 times their U rows (K = 64), so each trailing element is
+//      read and written once per 64 columns.
+// The columns left of a block are not read again, so the interchanges of
+// the later blocks reach them once, at the end: per 32 columns, the rows
+// below the block staged in LDS and written to their final rows.
 // Every global access is to (row, column) of the n x n matrix with both
 // below n (the caller checks lda >= n), every LDS access below its bounds.
 template <int PREC>
@@ -465,11 +471,14 @@ __device__ void lu_matrix(float* A, long long lda, int n, int* ipiv, int* info, 
     __shared__ int rowmap[kLuMaxOrder];         // panel rows: whose content ends where
     __shared__ int piv_all[kLuMaxOrder];        // every column's pivot row (absolute)
     __shared__ int mv_src[2 * kNB], mv_dst[2 * kNB];
+    __shared__ float lba[kNB][kNB + 1];         // the first panel's rows of the second panel (L_ba)
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     constexpr float sfmin = FLT_MIN;
     for (int k = 0; k < n; k += kNB) {
         const int kb = imin(kNB, n - k);
         const int m = n - k;
+        const bool second = (k / kNB) & 1;     // the second panel of its block
+        const int k0 = second ? k - kNB : k;   // the block's first column
         // 1. panel -> LDS: thread tid takes rows tid, tid + 256, ..., each
         //    row's 32 loads in flight together (coalesced across the wave:
         //    consecutive rows of one column); columns past kb are zero
@@ -586,9 +595,9 @@ __device__ void lu_matrix(float* A, long long lda, int n, int* ipiv, int* info, 
                 if (c + 1 < kb) p += lda;
             }
         }
-        const int c1 = k + kb;
-        const int mr = n - c1;
-        if (mr <= 0) continue;   // the last panel (kb = 32 below)
+        const int c1 = k + kb;           // the first column right of the panel
+        const bool right = c1 < n;       // (then kb = 32)
+        if (!right && !second) continue;
         // the panel's interchanges, composed: rowmap[r] = the row whose
         // content ends at row r, for r in {0..kb-1} and the pivot rows
         if (tid < kb) {
@@ -627,95 +636,150 @@ __device__ void lu_matrix(float* A, long long lda, int n, int* ipiv, int* info, 
             mv_src[tid] = (dst >= 0 && src != dst) ? src : -1;
         }
         __syncthreads();
-        // 3. the columns right of the panel, one per thread in registers:
-        //    the interchanges (rows 0..31 read from their sources, the pivot
-        //    rows below written from theirs: every read before any write),
-        //    then U12 := L11^{-1} A12 (unit lower L11 = pnl rows 0..31,
-        //    read as workgroup-uniform LDS broadcasts)
-        for (int c = c1 + tid; c < n; c += 256) {
-            float* p = A + off(k, c, lda);
-            float x[kNB], y[kNB];
+        if (second) {
+            // 2b. the moves on the first panel's 32 columns (one per
+            //     thread; every read before any write); their rows of this
+            //     panel (L_ba) to LDS for step 3
+            if (tid < kNB) {
+                float* p = A + off(k, k0 + tid, lda);
+                float x[kNB], y[kNB];
 #pragma unroll
-            for (int i = 0; i < kNB; ++i) x[i] = p[mv_src[i] >= 0 ? mv_src[i] : i];
+                for (int i = 0; i < kNB; ++i) x[i] = i < kb ? p[mv_src[i] >= 0 ? mv_src[i] : i] : 0.0f;
 #pragma unroll
-            for (int u = 0; u < kNB; ++u) y[u] = mv_src[kNB + u] >= 0 ? p[mv_src[kNB + u]] : 0.0f;
+                for (int u = 0; u < kNB; ++u) y[u] = mv_src[kNB + u] >= 0 ? p[mv_src[kNB + u]] : 0.0f;
 #pragma unroll
-            for (int u = 0; u < kNB; ++u)
-                if (mv_src[kNB + u] >= 0) p[mv_dst[kNB + u]] = y[u];
+                for (int u = 0; u < kNB; ++u)
+                    if (mv_src[kNB + u] >= 0) p[mv_dst[kNB + u]] = y[u];
 #pragma unroll
-            for (int i = 1; i < kNB; ++i) {
-                float v = x[i];
-#pragma unroll
-                for (int q = 0; q < i; ++q) v -= pnl[i][q] * x[q];
-                x[i] = v;
-                // row i + 1 of L11 is read after x[i] is done (all 496
-                // entries ahead of the arithmetic would spill)
-                asm volatile("" : "+v"(x[i]) : : "memory");
+                for (int i = 0; i < kNB; ++i) {
+                    if (mv_src[i] >= 0) p[i] = x[i];
+                    lba[i][tid] = x[i];
+                }
             }
-#pragma unroll
-            for (int i = 0; i < kNB; ++i) p[i] = x[i];
+            __syncthreads();
         }
-        __syncthreads();
-        // 4. A22 -= L21 U12
-        float* C = A + off(c1, c1, lda);
-        const float* L21 = A + off(c1, k, lda);
-        const float* U12 = A + off(k, c1, lda);
-        auto la = [&](int i, int kk) { return L21[off(i, kk, lda)]; };
-        auto rb = [&](int kk, int jc) { return U12[off(kk, jc, lda)]; };
-        if (mr <= small_max) {
-            // the panel array is free once U12 is solved
-            tile_update_small<PREC, false>(C, lda, mr, mr, kb, la, rb, &pnl[0][0]);
+        if (right) {
+            // 3. the columns right of the panel, one per thread in
+            //    registers: rows 0..31 read from their sources, the pivot
+            //    rows below written from theirs (every read before any
+            //    write); second panel: minus L_ba times the first panel's U
+            //    rows of the column; then U := L11^{-1} (.) (unit lower L11
+            //    = pnl rows 0..31, read as workgroup-uniform LDS broadcasts)
+            for (int c = c1 + tid; c < n; c += 256) {
+                float* p = A + off(k, c, lda);
+                float x[kNB], y[kNB];
+#pragma unroll
+                for (int i = 0; i < kNB; ++i) x[i] = p[mv_src[i] >= 0 ? mv_src[i] : i];
+#pragma unroll
+                for (int u = 0; u < kNB; ++u) y[u] = mv_src[kNB + u] >= 0 ? p[mv_src[kNB + u]] : 0.0f;
+#pragma unroll
+                for (int u = 0; u < kNB; ++u)
+                    if (mv_src[kNB + u] >= 0) p[mv_dst[kNB + u]] = y[u];
+                if (second) {
+                    const float* pa = A + off(k0, c, lda);
+                    float xa[kNB];
+#pragma unroll
+                    for (int q = 0; q < kNB; ++q) xa[q] = pa[q];
+#pragma unroll
+                    for (int i = 0; i < kNB; ++i) {
+                        float v = x[i];
+#pragma unroll
+                        for (int q = 0; q < kNB; ++q) v -= lba[i][q] * xa[q];
+                        x[i] = v;
+                        // row i + 1 of L_ba is read after x[i] is done
+                        asm volatile("" : "+v"(x[i]) : : "memory");
+                    }
+                }
+#pragma unroll
+                for (int i = 1; i < kNB; ++i) {
+                    float v = x[i];
+#pragma unroll
+                    for (int q = 0; q < i; ++q) v -= pnl[i][q] * x[q];
+                    x[i] = v;
+                    // row i + 1 of L11 is read after x[i] is done (all 496
+                    // entries ahead of the arithmetic would spill)
+                    asm volatile("" : "+v"(x[i]) : : "memory");
+                }
+#pragma unroll
+                for (int i = 0; i < kNB; ++i) p[i] = x[i];
+            }
+            __syncthreads();
+        }
+        // 4. the updates (the panel array is free once step 3 is done)
+        if (!second) {
+            // the second panel's columns by this panel (a 32-wide strip);
+            // the columns after them wait for the second panel
+            const int r1 = k + kNB;
+            const int ms = n - r1;
+            float* C = A + off(r1, r1, lda);
+            const float* L21 = A + off(r1, k, lda);
+            const float* U12 = A + off(k, r1, lda);
+            auto la = [&](int i, int kk) { return L21[off(i, kk, lda)]; };
+            auto rb = [&](int kk, int jc) { return U12[off(kk, jc, lda)]; };
+            if (ms <= small_max) tile_update_small<PREC, false>(C, lda, ms, imin(kNB, ms), kNB, la, rb, &pnl[0][0]);
+            else tile_sweep<PREC, false, 1>(C, lda, ms, imin(kNB, ms), kNB, la, rb, &pnl[0][0]);
             continue;
         }
-        // the panel array is free once U12 is solved
-        tile_sweep<PREC, false, 1>(C, lda, mr, mr, kb, la, rb, &pnl[0][0]);
+        if (!right) continue;
+        // the trailing matrix by both panels of the block (K = 64)
+        const int mr = n - c1;
+        float* C = A + off(c1, c1, lda);
+        const float* L21 = A + off(c1, k0, lda);
+        const float* U12 = A + off(k0, c1, lda);
+        auto la = [&](int i, int kk) { return L21[off(i, kk, lda)]; };
+        auto rb = [&](int kk, int jc) { return U12[off(kk, jc, lda)]; };
+        if (mr <= small_max) tile_update_small<PREC, false>(C, lda, mr, mr, 2 * kNB, la, rb, &pnl[0][0]);
+        else tile_sweep<PREC, false, 2>(C, lda, mr, mr, 2 * kNB, la, rb, &pnl[0][0]);
     }
-    // 5. the interchanges of the later panels, applied to the columns of
-    //    each earlier panel once at the end (no step reads those columns
-    //    after their panel): inv = the composition of the later panels'
-    //    exchanges (rowmap), built backwards; rows [kn, n) of the panel's
-    //    columns staged in LDS, then each row written to inv(row)
-    const int panels = (n + kNB - 1) / kNB;
-    if (panels > 1) {
+    // 5. the interchanges of the later blocks, applied to the columns of
+    //    each earlier block once at the end (no step reads those columns
+    //    after their block): inv = the composition of the later blocks'
+    //    exchanges (rowmap), built backwards; per 32 columns, rows [kn, n)
+    //    staged in LDS, then each moved row written to inv(row)
+    const int blocks = (n + 2 * kNB - 1) / (2 * kNB);
+    if (blocks > 1) {
         for (int r = tid; r < n; r += 256) rowmap[r] = r;
         __syncthreads();
-        for (int b = panels - 2; b >= 0; --b) {
-            const int kn = (b + 1) * kNB;
+        for (int b = blocks - 2; b >= 0; --b) {
+            const int kn = (b + 1) * 2 * kNB;
             if (tid == 0) {
-                for (int t = imin(kNB, n - kn) - 1; t >= 0; --t) {
-                    const int j = kn + t, p = piv_all[j];
+                for (int j = imin(kn + 2 * kNB, n) - 1; j >= kn; --j) {
+                    const int p = piv_all[j];
                     const int v = rowmap[j];
                     rowmap[j] = rowmap[p];
                     rowmap[p] = v;
                 }
             }
             __syncthreads();
-            for (int r = kn + tid; r < n; r += 256) {
-                const float* q = A + off(r, b * kNB, lda);
-                float v[kNB];
+            for (int h = 0; h < 2; ++h) {
+                const int cb = b * 2 * kNB + h * kNB;
+                for (int r = kn + tid; r < n; r += 256) {
+                    const float* q = A + off(r, cb, lda);
+                    float v[kNB];
 #pragma unroll
-                for (int c = 0; c < kNB; ++c) {
-                    v[c] = *q;
-                    if (c + 1 < kNB) q += lda;
+                    for (int c = 0; c < kNB; ++c) {
+                        v[c] = *q;
+                        if (c + 1 < kNB) q += lda;
+                    }
+#pragma unroll
+                    for (int c = 0; c < kNB; ++c) pnl[r - kn][c] = v[c];
                 }
+                __syncthreads();
+                for (int r = kn + tid; r < n; r += 256) {
+                    const int d = rowmap[r];
+                    if (d == r) continue;
+                    float* q = A + off(d, cb, lda);
+                    float v[kNB];
 #pragma unroll
-                for (int c = 0; c < kNB; ++c) pnl[r - kn][c] = v[c];
-            }
-            __syncthreads();
-            for (int r = kn + tid; r < n; r += 256) {
-                const int d = rowmap[r];
-                if (d == r) continue;
-                float* q = A + off(d, b * kNB, lda);
-                float v[kNB];
+                    for (int c = 0; c < kNB; ++c) v[c] = pnl[r - kn][c];
 #pragma unroll
-                for (int c = 0; c < kNB; ++c) v[c] = pnl[r - kn][c];
-#pragma unroll
-                for (int c = 0; c < kNB; ++c) {
-                    *q = v[c];
-                    if (c + 1 < kNB) q += lda;
+                    for (int c = 0; c < kNB; ++c) {
+                        *q = v[c];
+                        if (c + 1 < kNB) q += lda;
+                    }
                 }
+                __syncthreads();
             }
-            __syncthreads();
         }
     }
 }

@@ -88,60 +88,67 @@ struct Lds {
 
 using Panel = Lds<float, kLuMaxOrder, kNB + 1>;
 
-// A22 -= L21 U12 tile by tile (64 x 64, down each tile column, then the
-// next), the operands of tile t staged in buffer t % 2 of the panel array
-// (k-major, 2 x kNB x 65 floats per buffer); each tile's C block is read,
-// then written (the products run as C^T -= R^T L^T, so in the MFMA
-// accumulator layout lane l holds row l % 32, columns 8 (r / 4) + 4 (l / 32)
-// + r % 4).
-void tile_sweep(Checked& M, Panel& pnl, int c1, int k, int m, int kb)
+// C -= L R with C = A(r1.., r1..) m x nc, L = A(r1.., kL..) and R =
+// A(kL.., r1..), K = kw (<= 64), tile by tile (64 x 64, down each tile
+// column, then the next), each tile in 32-wide k slices; the operands of
+// step s staged in buffer s % 2 of the panel array (k-major, 2 x kNB x 65
+// floats per buffer); each tile's C block is read at its first slice and
+// written after its last (the products run as C^T -= R^T L^T, so in the
+// MFMA accumulator layout lane l holds row l % 32, columns 8 (r / 4) + 4
+// (l / 32) + r % 4).
+void tile_sweep(Checked& M, Panel& pnl, int r1, int kL, int m, int nc, int kw)
 {
-    const int tiles = (m + kT - 1) / kT;
+    const int tiles_r = (m + kT - 1) / kT, tiles_c = (nc + kT - 1) / kT;
+    const int slices = (kw + kNB - 1) / kNB;
     constexpr int kTileFloats = 2 * kNB * (kT + 1);
     auto buf = [&](int t, int half, int kk, int i) -> float& {
         const int idx = (t & 1) * kTileFloats + half * kNB * (kT + 1) + kk * (kT + 1) + i;
         return pnl.at(idx / (kNB + 1), idx % (kNB + 1));
     };
+    std::vector<float> acc(static_cast<size_t>(kThreads) * 16), cv(acc.size());
     int t = 0;
-    for (int tj = 0; tj < tiles; ++tj)
-        for (int ti = 0; ti < tiles; ++ti, ++t) {
-            const int r0 = ti * kT, c0 = tj * kT;
-            for (int tid = 0; tid < kThreads; ++tid)
-                for (int q = 0; q < kNB * kT / kThreads; ++q) {
-                    const int e = tid + kThreads * q;
-                    const int kk = e / kT, i = e % kT;
-                    float va = 0.0f, vb = 0.0f;
-                    if (kk < kb) {
-                        if (r0 + i < m) va = M.A(c1 + r0 + i, k + kk);
-                        if (c0 + i < m) vb = M.A(k + kk, c1 + c0 + i);
+    for (int tj = 0; tj < tiles_c; ++tj)
+        for (int ti = 0; ti < tiles_r; ++ti)
+            for (int sl = 0; sl < slices; ++sl, ++t) {
+                const int r0 = ti * kT, c0 = tj * kT, k0 = sl * kNB;
+                for (int tid = 0; tid < kThreads; ++tid)
+                    for (int q = 0; q < kNB * kT / kThreads; ++q) {
+                        const int e = tid + kThreads * q;
+                        const int kk = k0 + e / kT, i = e % kT;
+                        float va = 0.0f, vb = 0.0f;
+                        if (kk < kw) {
+                            if (r0 + i < m) va = M.A(r1 + r0 + i, kL + kk);
+                            if (c0 + i < nc) vb = M.A(kL + kk, r1 + c0 + i);
+                        }
+                        buf(t, 0, e / kT, i) = va;
+                        buf(t, 1, e / kT, i) = vb;
                     }
-                    buf(t, 0, kk, i) = va;
-                    buf(t, 1, kk, i) = vb;
-                }
-            for (int tid = 0; tid < kThreads; ++tid) {
-                const int lane = tid & 63, wave = tid >> 6, wm = wave & 1, wn = wave >> 1;
-                const int row = wm * 32 + (lane & 31);
-                const int gr = r0 + row;
-                if (gr >= m) continue;
-                for (int r = 0; r < 16; ++r) {
-                    const int lc = wn * 32 + 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3);
-                    if (c0 + lc >= m) continue;
-                    const float cv = M.A(c1 + gr, c1 + c0 + lc);
-                    float acc = 0.0f;
-                    for (int kk = 0; kk < kNB; ++kk)
-                        acc = std::fma(buf(t, 1, kk, lc), buf(t, 0, kk, row), acc);
-                    M.A(c1 + gr, c1 + c0 + lc) = cv - acc;
+                for (int tid = 0; tid < kThreads; ++tid) {
+                    const int lane = tid & 63, wave = tid >> 6, wm = wave & 1, wn = wave >> 1;
+                    const int row = wm * 32 + (lane & 31);
+                    const int gr = r0 + row;
+                    for (int r = 0; r < 16; ++r) {
+                        const int lc = wn * 32 + 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3);
+                        const bool ok = gr < m && c0 + lc < nc;
+                        float& a = acc[tid * 16 + r];
+                        if (sl == 0) {
+                            cv[tid * 16 + r] = ok ? M.A(r1 + gr, r1 + c0 + lc) : 0.0f;
+                            a = 0.0f;
+                        }
+                        for (int kk = 0; kk < kNB; ++kk) a = std::fma(buf(t, 1, kk, lc), buf(t, 0, kk, row), a);
+                        if (sl == slices - 1 && ok) M.A(r1 + gr, r1 + c0 + lc) = cv[tid * 16 + r] - a;
+                    }
                 }
             }
-        }
 }
 
 // The small size class: wave w takes tiles w, w + 4, ... of the 32-blocks of
-// the trailing matrix (column-major tile order), stages its operands in its
-// 2 x 32 x 33 slice of the panel array and writes its 32 x 32 tile.
-void tile_update_small(Checked& M, Panel& pnl, int c1, int k, int m, int kb)
+// C (column-major tile order), stages its operands one 32-wide k slice at a
+// time in its 2 x 32 x 33 slice of the panel array and writes its 32 x 32
+// tile after the last slice.
+void tile_update_small(Checked& M, Panel& pnl, int r1, int kL, int m, int nc, int kw)
 {
-    const int tb = (m + kSmallT - 1) / kSmallT, count = tb * tb;
+    const int tr = (m + kSmallT - 1) / kSmallT, tc = (nc + kSmallT - 1) / kSmallT, count = tr * tc;
     auto slot = [&](int wave, int half, int kk, int i) -> float& {
         const int idx = wave * 2 * kNB * (kSmallT + 1) + half * kNB * (kSmallT + 1)
                         + kk * (kSmallT + 1) + i;
@@ -149,27 +156,34 @@ void tile_update_small(Checked& M, Panel& pnl, int c1, int k, int m, int kb)
     };
     for (int wave = 0; wave < 4; ++wave)
         for (int t = wave; t < count; t += 4) {
-            const int r0 = (t % tb) * kSmallT, c0 = (t / tb) * kSmallT;
-            for (int lane = 0; lane < 64; ++lane)
-                for (int e = lane; e < kNB * kSmallT; e += 64) {
-                    const int i = e % kSmallT, kk = e / kSmallT;
-                    float va = 0.0f, vb = 0.0f;
-                    if (kk < kb) {
-                        if (r0 + i < m) va = M.A(c1 + r0 + i, k + kk);
-                        if (c0 + i < m) vb = M.A(k + kk, c1 + c0 + i);
+            const int r0 = (t % tr) * kSmallT, c0 = (t / tr) * kSmallT;
+            std::vector<float> acc(64 * 16, 0.0f);
+            for (int k0 = 0; k0 < kw; k0 += kNB) {
+                for (int lane = 0; lane < 64; ++lane)
+                    for (int e = lane; e < kNB * kSmallT; e += 64) {
+                        const int i = e % kSmallT, kk = e / kSmallT;
+                        float va = 0.0f, vb = 0.0f;
+                        if (k0 + kk < kw) {
+                            if (r0 + i < m) va = M.A(r1 + r0 + i, kL + k0 + kk);
+                            if (c0 + i < nc) vb = M.A(kL + k0 + kk, r1 + c0 + i);
+                        }
+                        slot(wave, 0, kk, i) = va;
+                        slot(wave, 1, kk, i) = vb;
                     }
-                    slot(wave, 0, kk, i) = va;
-                    slot(wave, 1, kk, i) = vb;
-                }
+                for (int lane = 0; lane < 64; ++lane)
+                    for (int r = 0; r < 16; ++r) {
+                        const int lc = 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3);
+                        for (int kk = 0; kk < kNB; ++kk)
+                            acc[lane * 16 + r] = std::fma(slot(wave, 1, kk, lc), slot(wave, 0, kk, lane & 31),
+                                                          acc[lane * 16 + r]);
+                    }
+            }
             for (int lane = 0; lane < 64; ++lane) {
-                const int rc = lane & 31, gr = r0 + rc;
+                const int gr = r0 + (lane & 31);
                 if (gr >= m) continue;
                 for (int r = 0; r < 16; ++r) {
                     const int lc = 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3);
-                    float acc = 0.0f;
-                    for (int kk = 0; kk < kNB; ++kk)
-                        acc = std::fma(slot(wave, 1, kk, lc), slot(wave, 0, kk, rc), acc);
-                    if (c0 + lc < m) M.A(c1 + gr, c1 + c0 + lc) -= acc;
+                    if (c0 + lc < nc) M.A(r1 + gr, r1 + c0 + lc) -= acc[lane * 16 + r];
                 }
             }
         }
@@ -189,9 +203,13 @@ void lu_matrix(Checked& M, int n, int* info)
     Lds<int, 2, 4> cand_i;
     cand.violations = cand_v.violations = cand_i.violations = &M.violations;
     pnl.violations = piv_s.violations = &M.violations;
+    static Lds<float, kNB, kNB + 1> lba;
+    lba.violations = &M.violations;
     for (int k = 0; k < n; k += kNB) {
         const int kb = std::min(kNB, n - k);
         const int m = n - k;
+        const bool second = (k / kNB) & 1;
+        const int k0 = second ? k - kNB : k;
         // 1. panel -> LDS, rows tid, tid + 256, ... per thread
         for (int tid = 0; tid < kThreads; ++tid)
             for (int r = tid; r < m; r += kThreads)
@@ -293,11 +311,12 @@ void lu_matrix(Checked& M, int n, int* info)
         for (int tid = 0; tid < kThreads; ++tid)
             for (int r = tid; r < m; r += kThreads)
                 for (int c = 0; c < kb; ++c) M.A(k + r, k + c) = pnl.at(r, c);
-        const int c1 = k + kb, mr = n - c1;
-        if (mr <= 0) continue;
+        const int c1 = k + kb;
+        const bool right = c1 < n;
+        if (!right && !second) continue;
         // the panel's interchanges composed into a move list (rowmap: whose
         // content ends at each panel row)
-        if (kb != kNB) M.report("right columns after a narrow panel", k, kb);
+        if (right && kb != kNB) M.report("right columns after a narrow panel", k, kb);
         for (int tid = 0; tid < kb; ++tid) {
             rowmap.at(0, tid) = tid;
             rowmap.at(0, piv_s.at(0, tid)) = piv_s.at(0, tid);
@@ -321,49 +340,85 @@ void lu_matrix(Checked& M, int n, int* info)
             mv_dst.at(0, tid) = dst;
             mv_src.at(0, tid) = (dst >= 0 && src != dst) ? src : -1;
         }
-        // 3. each column right of the panel: rows 0..31 read from their
-        //    sources, the pivot rows below written from theirs (every read
-        //    before any write), then U12 := L11^{-1} A12 in registers
-        for (int c = c1; c < n; ++c) {
-            float x[kNB], y[kNB];
-            for (int ii = 0; ii < kNB; ++ii)
-                x[ii] = M.A(k + (mv_src.at(0, ii) >= 0 ? mv_src.at(0, ii) : ii), c);
-            for (int u = 0; u < kNB; ++u)
-                y[u] = mv_src.at(0, kNB + u) >= 0 ? M.A(k + mv_src.at(0, kNB + u), c) : 0.0f;
-            for (int u = 0; u < kNB; ++u)
-                if (mv_src.at(0, kNB + u) >= 0) M.A(k + mv_dst.at(0, kNB + u), c) = y[u];
-            for (int ii = 1; ii < kNB; ++ii) {
-                float v = x[ii];
-                for (int q = 0; q < ii; ++q) v -= pnl.at(ii, q) * x[q];
-                x[ii] = v;
+        if (second) {
+            // 2b. the moves on the first panel's columns; L_ba to LDS
+            for (int tid = 0; tid < kNB; ++tid) {
+                const int c = k0 + tid;
+                float x[kNB], y[kNB];
+                for (int ii = 0; ii < kNB; ++ii)
+                    x[ii] = ii < kb ? M.A(k + (mv_src.at(0, ii) >= 0 ? mv_src.at(0, ii) : ii), c) : 0.0f;
+                for (int u = 0; u < kNB; ++u)
+                    y[u] = mv_src.at(0, kNB + u) >= 0 ? M.A(k + mv_src.at(0, kNB + u), c) : 0.0f;
+                for (int u = 0; u < kNB; ++u)
+                    if (mv_src.at(0, kNB + u) >= 0) M.A(k + mv_dst.at(0, kNB + u), c) = y[u];
+                for (int ii = 0; ii < kNB; ++ii) {
+                    if (mv_src.at(0, ii) >= 0) M.A(k + ii, c) = x[ii];
+                    lba.at(ii, tid) = x[ii];
+                }
             }
-            for (int ii = 0; ii < kNB; ++ii) M.A(k + ii, c) = x[ii];
         }
-        // 4. A22 -= L21 U12
-        if (mr <= small_max) {
-            tile_update_small(M, pnl, c1, k, mr, kb);
+        if (right) {
+            // 3. each column right of the panel: the moves (every read
+            //    before any write), (second panel) minus L_ba times the first
+            //    panel's U rows, then U := L11^{-1} (.) in registers
+            for (int c = c1; c < n; ++c) {
+                float x[kNB], y[kNB];
+                for (int ii = 0; ii < kNB; ++ii)
+                    x[ii] = M.A(k + (mv_src.at(0, ii) >= 0 ? mv_src.at(0, ii) : ii), c);
+                for (int u = 0; u < kNB; ++u)
+                    y[u] = mv_src.at(0, kNB + u) >= 0 ? M.A(k + mv_src.at(0, kNB + u), c) : 0.0f;
+                for (int u = 0; u < kNB; ++u)
+                    if (mv_src.at(0, kNB + u) >= 0) M.A(k + mv_dst.at(0, kNB + u), c) = y[u];
+                if (second) {
+                    float xa[kNB];
+                    for (int q = 0; q < kNB; ++q) xa[q] = M.A(k0 + q, c);
+                    for (int ii = 0; ii < kNB; ++ii) {
+                        float v = x[ii];
+                        for (int q = 0; q < kNB; ++q) v -= lba.at(ii, q) * xa[q];
+                        x[ii] = v;
+                    }
+                }
+                for (int ii = 1; ii < kNB; ++ii) {
+                    float v = x[ii];
+                    for (int q = 0; q < ii; ++q) v -= pnl.at(ii, q) * x[q];
+                    x[ii] = v;
+                }
+                for (int ii = 0; ii < kNB; ++ii) M.A(k + ii, c) = x[ii];
+            }
+        }
+        // 4. first panel: the strip of the second panel's columns; second
+        //    panel: the trailing matrix by both (K = 64)
+        if (!second) {
+            const int r1 = k + kNB, ms = n - r1;
+            if (ms <= small_max) tile_update_small(M, pnl, r1, k, ms, std::min(kNB, ms), kNB);
+            else tile_sweep(M, pnl, r1, k, ms, std::min(kNB, ms), kNB);
             continue;
         }
-        tile_sweep(M, pnl, c1, k, mr, kb);
+        if (!right) continue;
+        const int mr = n - c1;
+        if (mr <= small_max) tile_update_small(M, pnl, c1, k0, mr, mr, 2 * kNB);
+        else tile_sweep(M, pnl, c1, k0, mr, mr, 2 * kNB);
     }
-    // 5. the later panels' interchanges applied to each earlier panel's
-    //    columns at the end: inv (rowmap) composed backwards, rows [kn, n)
-    //    staged in the panel array, each written to inv(row)
-    const int panels = (n + kNB - 1) / kNB;
-    if (panels > 1) {
+    // 5. the later blocks' interchanges applied to each earlier block's
+    //    columns at the end: inv (rowmap) composed backwards; per 32
+    //    columns, rows [kn, n) staged in the panel array, each moved row
+    //    written to inv(row)
+    const int blocks = (n + 2 * kNB - 1) / (2 * kNB);
+    if (blocks > 1) {
         for (int r = 0; r < n; ++r) rowmap.at(0, r) = r;
-        for (int b = panels - 2; b >= 0; --b) {
-            const int kn = (b + 1) * kNB;
-            for (int t = std::min(kNB, n - kn) - 1; t >= 0; --t) {
-                const int j = kn + t, pp = piv_all.at(0, j);
-                std::swap(rowmap.at(0, j), rowmap.at(0, pp));
-            }
-            for (int r = kn; r < n; ++r)
-                for (int c = 0; c < kNB; ++c) pnl.at(r - kn, c) = M.A(r, b * kNB + c);
-            for (int r = kn; r < n; ++r) {
-                const int d = rowmap.at(0, r);
-                if (d == r) continue;
-                for (int c = 0; c < kNB; ++c) M.A(d, b * kNB + c) = pnl.at(r - kn, c);
+        for (int b = blocks - 2; b >= 0; --b) {
+            const int kn = (b + 1) * 2 * kNB;
+            for (int j = std::min(kn + 2 * kNB, n) - 1; j >= kn; --j)
+                std::swap(rowmap.at(0, j), rowmap.at(0, piv_all.at(0, j)));
+            for (int h = 0; h < 2; ++h) {
+                const int cb = b * 2 * kNB + h * kNB;
+                for (int r = kn; r < n; ++r)
+                    for (int c = 0; c < kNB; ++c) pnl.at(r - kn, c) = M.A(r, cb + c);
+                for (int r = kn; r < n; ++r) {
+                    const int d = rowmap.at(0, r);
+                    if (d == r) continue;
+                    for (int c = 0; c < kNB; ++c) M.A(d, cb + c) = pnl.at(r - kn, c);
+                }
             }
         }
     }
