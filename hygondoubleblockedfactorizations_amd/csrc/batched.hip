@@ -306,6 +306,8 @@ __device__ inline float lane_value(float v, int lane)
 // A last panel narrower than 32 (kb < 32) reads its column kb - 1 (and, in
 // the POTF2, its row kb - 1) in place of the missing ones: every access
 // stays inside the matrix and nothing computed from the copies is stored.
+constexpr int kCholPanels = 4;   // 32-column panels per outer block
+
 template <int PREC>
 __device__ void chol_matrix(float* A, long long lda, int n, int* info, int small_max)
 {
@@ -402,25 +404,23 @@ __device__ void chol_matrix(float* A, long long lda, int n, int* info, int small
         auto la = [&](int i, int kk) { return Pp[off(i, kk, lda)]; };
         auto rb = [&](int kk, int j) { return Pp[off(j, kk, lda)]; };
         if (m <= small_max) tile_update_small<PREC, true>(C, lda, m, nc, kw, la, rb, scratch);
-        else tile_sweep<PREC, true, 2>(C, lda, m, nc, kw, la, rb, scratch);
+        else tile_sweep<PREC, true, kCholPanels>(C, lda, m, nc, kw, la, rb, scratch);
     };
-    // per outer block: panel a, the strip of panel b by panel a, panel b,
-    // the trailing matrix by both (a runtime loop: one copy of each step's
-    // code)
-    for (int k = 0; k < n; k += 2 * kNB) {
-        const int kb2 = imin(2 * kNB, n - k);
+    // per outer block of kCholPanels panels: before panel q its 32 columns
+    // by panels 0 .. q - 1 of the block (a strip, K = 32 q), then the panel;
+    // after the last one the trailing matrix by the whole block (a runtime
+    // loop: one copy of each step's code)
+    for (int k = 0; k < n; k += kCholPanels * kNB) {
+        const int kbb = imin(kCholPanels * kNB, n - k);
+        const int nq = (kbb + kNB - 1) / kNB;
 #pragma nounroll
-        for (int step = 0; step < 4; ++step) {
-            if (step & 1) {
-                const bool strip = step == 1;
-                const int r1 = strip ? k + kNB : k + kb2;
-                if (r1 >= n || (strip && kb2 <= kNB)) continue;
-                update(r1, strip ? kb2 - kNB : n - r1, k, strip ? kNB : kb2);
-            } else {
-                const int kp = k + (step >> 1) * kNB;
-                if (kp >= k + kb2) continue;
-                if (!panel(kp, imin(kNB, k + kb2 - kp))) return;
+        for (int q = 0; q <= nq; ++q) {
+            const int kp = imin(k + q * kNB, k + kbb);
+            if (q > 0 && kp < n) {
+                const bool strip = q < nq;
+                update(kp, strip ? imin(kNB, k + kbb - kp) : n - kp, k, kp - k);
             }
+            if (q < nq && !panel(kp, imin(kNB, k + kbb - kp))) return;
         }
     }
 }

@@ -1,7 +1,7 @@
 // Host replay of the batched Cholesky kernel's address arithmetic
 // (hygondoubleblockedfactorizations_amd/csrc/batched.hip: order_kernel,
-// batched_kernel<kPrecFP32, false>, chol_matrix in 64-column outer blocks of
-// two 32-column panels, tile_sweep<kPrecFP32, true> with its k slices and
+// batched_kernel<kPrecFP32, false>, chol_matrix in 128-column outer blocks of
+// four 32-column panels, tile_sweep<kPrecFP32, true> with its k slices and
 // two operand buffers and, for trailing orders up to the small size class,
 // tile_update_small<kPrecFP32, true>).
 //
@@ -30,7 +30,7 @@
 namespace {
 
 constexpr int kNB = 32, kT = 64, kMaxOrder = 4096, kBuckets = 256, kThreads = 256,
-              kSmallT = 32;
+              kSmallT = 32, kPanels = 4;
 constexpr int kTileFloats = 2 * kNB * (kT + 1);
 constexpr int kSmallScratch = 4 * 2 * kNB * (kSmallT + 1);
 constexpr int kScratch = kSmallScratch > 2 * kTileFloats ? kSmallScratch : 2 * kTileFloats;
@@ -256,7 +256,7 @@ bool panel(Checked& M, Shared& S, int n, int k, int kb, int* info)
 void update(Checked& M, Shared& S, int n, int r1, int nc, int k, int kw)
 {
     const int m = n - r1;
-    if (nc <= 0 || nc > m || kw <= 0 || kw > 2 * kNB) M.report("update shape", nc, kw);
+    if (nc <= 0 || nc > m || kw <= 0 || kw > kPanels * kNB) M.report("update shape", nc, kw);
     if (m <= small_max) tile_update_small(M, S, r1, m, nc, k, kw);
     else tile_sweep(M, S, r1, m, nc, k, kw);
 }
@@ -265,19 +265,18 @@ void chol_matrix(Checked& M, int n, int* info)
 {
     static Shared S;
     S.violations = &M.violations;
-    for (int k = 0; k < n; k += 2 * kNB) {
-        const int kb2 = imin(2 * kNB, n - k);
-        for (int step = 0; step < 4; ++step) {
-            if (step & 1) {
-                const bool strip = step == 1;
-                const int r1 = strip ? k + kNB : k + kb2;
-                if (r1 >= n || (strip && kb2 <= kNB)) continue;
-                update(M, S, n, r1, strip ? kb2 - kNB : n - r1, k, strip ? kNB : kb2);
-            } else {
-                const int kp = k + (step >> 1) * kNB;
-                if (kp >= k + kb2) continue;
-                if (!panel(M, S, n, kp, imin(kNB, k + kb2 - kp), info)) return;
+    // outer blocks of kPanels panels: before panel q its columns by panels
+    // 0 .. q - 1 of the block, after the last the trailing matrix
+    for (int k = 0; k < n; k += kPanels * kNB) {
+        const int kbb = imin(kPanels * kNB, n - k);
+        const int nq = (kbb + kNB - 1) / kNB;
+        for (int q = 0; q <= nq; ++q) {
+            const int kp = imin(k + q * kNB, k + kbb);
+            if (q > 0 && kp < n) {
+                const bool strip = q < nq;
+                update(M, S, n, kp, strip ? imin(kNB, k + kbb - kp) : n - kp, k, kp - k);
             }
+            if (q < nq && !panel(M, S, n, kp, imin(kNB, k + kbb - kp), info)) return;
         }
     }
 }
