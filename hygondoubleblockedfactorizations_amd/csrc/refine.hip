@@ -48,8 +48,8 @@ struct SolveArena {
     double* Z = nullptr;         // preconditioned directions, ldt x restart
     double* dpart = nullptr;     // dot-product partials
     double* dvals = nullptr;     // [h1 | h2 | ||w||^2 | y] on the device
-    int* flags = nullptr;        // TRSV block-ready flags (epoch tagged)
     unsigned long long* ticket = nullptr;  // TRSV ticket counter (monotonic)
+    unsigned long long* xtag = nullptr;    // TRSV solved blocks as (epoch, value) words, ldt x kRhsPass
 };
 
 int residual_chunks(int n)
@@ -102,15 +102,16 @@ size_t arena_bytes(int n, int nrhs, SolveArena* out, char* base)
     a.Z = reinterpret_cast<double*>(take(sizeof(double) * ldt * kGmresRestart));
     a.dpart = reinterpret_cast<double*>(take(sizeof(double) * kDotChunks * (kGmresRestart + 1)));
     a.dvals = reinterpret_cast<double*>(take(sizeof(double) * (4 * kGmresRestart + 8)));
-    a.flags = reinterpret_cast<int*>(take(sizeof(int) * blocks));
     a.ticket = reinterpret_cast<unsigned long long*>(take(sizeof(unsigned long long)));
+    a.xtag = reinterpret_cast<unsigned long long*>(
+        take(sizeof(unsigned long long) * ldt * static_cast<size_t>(kRhsPass)));
     if (out != nullptr) *out = a;
     return o;
 }
 
 // The arena's layout is a function of its capacity (largest order and
 // right-hand-side count requested so far), so every offset -- in particular
-// the TRSV flags and ticket counter -- stays put between calls until the
+// the TRSV tagged words and ticket counter -- stays put between calls until the
 // arena is reallocated (and zeroed) for a larger capacity.
 bool ensure_arena(mifact_handle h, int n, int nrhs, SolveArena* a)
 {
@@ -132,7 +133,7 @@ bool ensure_arena(mifact_handle h, int n, int nrhs, SolveArena* a)
             report_status(h, mifact_status_allocation_failed, "solver arena allocation failed");
             return false;
         }
-        // zero once: TRSV flags (epochs start at 1) and the ticket counter
+        // zero once: TRSV tagged words (epochs start at 1) and the ticket counter
         if (hipMemset(p, 0, need) != hipSuccess) {
             (void)hipFree(p);
             report_status(h, mifact_status_hip_error, "solver arena initialization failed");
@@ -547,8 +548,10 @@ __global__ void absmax_kernel(const double* x, int n, unsigned long long* out)
 // Triangular solves with few right-hand sides in one launch.  Row block i
 // (64 rows) is owned by the workgroup that draws ticket t (t = i forward,
 // nblk - 1 - i backward); it accumulates M_ij x_j over its dependency blocks
-// as their ready flags appear, then applies the 64 x 64 diagonal inverse and
-// publishes x_i with an epoch-tagged flag.  Tickets are handed out in
+// as they appear, then applies the 64 x 64 diagonal inverse and publishes
+// x_i as (epoch, value) words in `xtag` (ld `ldx`): a reader polls the words
+// it needs until every one carries the call's epoch, so no separate ready
+// flag (and no completion wait before one) is needed.  Tickets are handed out in
 // dispatch order, so every block waited on belongs to a workgroup that is
 // already running: the chain always progresses.  Waits are bounded; on
 // expiry the kernel raises the handle's status word and finishes.
@@ -559,7 +562,8 @@ constexpr int kTrsvSpin = 1 << 24;
 template <int MODE, int NR>
 __global__ __launch_bounds__(256) void trsv_kernel(const float* F, long long ldf, int n,
                                                    const float* inv, float* T, long long ldt,
-                                                   int nr, int* flags, int epoch,
+                                                   int nr, unsigned long long* xtag, int ldx,
+                                                   int epoch,
                                                    unsigned long long* ticket,
                                                    unsigned long long ticket_base, int* status,
                                                    int spin_limit)
@@ -607,22 +611,26 @@ __global__ __launch_bounds__(256) void trsv_kernel(const float* F, long long ldf
             const int lr = idx & 63, lc = idx >> 6;
             tile[lc][lr] = (lr < srows && lc < scols) ? F[off(sr + lr, sc + lc, ldf)] : 0.0f;
         }
-        // wait for x_j
-        if (tid == 0) {
-            int spins = 0;
-            while (__hip_atomic_load(flags + bj, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT)
-                   != epoch) {
-                if (++spins > spin_limit) {
-                    atomicExch(status, 3);
-                    break;
-                }
-                __builtin_amdgcn_s_sleep(1);
-            }
-        }
-        __syncthreads();
+        // x_j: each word polled until it carries this call's epoch
         for (int e = tid; e < 64 * NR; e += 256) {
             const int c = e / NR, k = e % NR;
-            xs[c][k] = (c < cb && k < nr) ? T[off(c0 + c, k, ldt)] : 0.0f;
+            float v = 0.0f;
+            if (c < cb && k < nr) {
+                unsigned long long* wp = xtag + static_cast<size_t>(k) * ldx + c0 + c;
+                int spins = 0;
+                unsigned long long word;
+                while (static_cast<int>((word = __hip_atomic_load(wp, __ATOMIC_RELAXED,
+                                                                  __HIP_MEMORY_SCOPE_AGENT)) >> 32)
+                       != epoch) {
+                    if (++spins > spin_limit) {
+                        atomicExch(status, 3);
+                        break;
+                    }
+                    __builtin_amdgcn_s_sleep(1);
+                }
+                v = __uint_as_float(static_cast<unsigned int>(word));
+            }
+            xs[c][k] = v;
         }
         __syncthreads();
 #pragma unroll 4
@@ -665,18 +673,17 @@ __global__ __launch_bounds__(256) void trsv_kernel(const float* F, long long ldf
     for (int e = tid; e < 64 * NR; e += 256) {
         const int r = e / NR, k = e % NR;
         if (r < rb && k < nr) {
-            // device-scope store: written through to the device's point of
-            // coherence, no dirty L2 line left behind
-            __hip_atomic_store(T + off(r0 + r, k, ldt),
-                               (part[0][r][k] + part[1][r][k]) + (part[2][r][k] + part[3][r][k]),
-                               __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            const float x = (part[0][r][k] + part[1][r][k]) + (part[2][r][k] + part[3][r][k]);
+            // the tagged word for the readers in this launch (device scope),
+            // the plain value for the caller
+            const unsigned long long word =
+                (static_cast<unsigned long long>(static_cast<unsigned int>(epoch)) << 32)
+                | __float_as_uint(x);
+            __hip_atomic_store(xtag + static_cast<size_t>(k) * ldx + r0 + r, word, __ATOMIC_RELAXED,
+                               __HIP_MEMORY_SCOPE_AGENT);
+            T[off(r0 + r, k, ldt)] = x;
         }
     }
-    // every wave's stores have completed at the device before the barrier,
-    // so the flag needs no L2 write-back (what a release fence would add)
-    __builtin_amdgcn_s_waitcnt(0);
-    __syncthreads();
-    if (tid == 0) __hip_atomic_store(flags + bi, epoch, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
 // ---------------------------------------------------------------- host pieces
@@ -828,7 +835,7 @@ void trsv(mifact_handle h, TrsvMode mode, int n, int nrhs, const float* F, long 
             constexpr int M = decltype(mode_tag)::value;
             constexpr int R = decltype(nr_tag)::value;
             hipLaunchKernelGGL((trsv_kernel<M, R>), dim3(nblk), dim3(256), 0, h->stream, F, ldf,
-                               n, inv, Tk, ldt, nr, w.flags, epoch, w.ticket, base,
+                               n, inv, Tk, ldt, nr, w.xtag, w.ldt, epoch, w.ticket, base,
                                h->kernel_status,
                                h->wait_limit > 0 && h->wait_limit < kTrsvSpin
                                    ? static_cast<int>(h->wait_limit) : kTrsvSpin);
