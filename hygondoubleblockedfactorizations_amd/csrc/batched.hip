@@ -427,6 +427,11 @@ __device__ void chol_matrix(float* A, long long lda, int n, int* info, int small
 
 // ---------------------------------------------------------------- LU
 
+// Wavefront max reduction of the device library (declared by the HIP
+// headers only under HIP_ENABLE_EXTRA_WARP_SYNC_TYPES).
+extern "C" __device__ __attribute__((const)) unsigned long long __ockl_wfred_max_u64(
+    unsigned long long);
+
 constexpr int kLuMaxOrder = 1024;   // the whole panel (up to 1024 x 32) sits in LDS
 constexpr int kPanelLd = kNB + 1;   // LDS row stride: conflict-free column walks
 
@@ -495,18 +500,25 @@ __device__ void lu_matrix(float* A, long long lda, int n, int* ipiv, int* info, 
         }
         __syncthreads();
         // wave arg max of (bv, bi) (first index among equal magnitudes; NaN
-        // never wins), published with a copy of the winning row; rows
-        // written by this wave's lanes in the same pass are seen (a wave's
-        // LDS accesses complete in order)
+        // never wins: bv only takes values that compared greater), published
+        // with a copy of the winning row; rows written by this wave's lanes
+        // in the same pass are seen (a wave's LDS accesses complete in
+        // order).  One wavefront max reduction of the key (|v| bits, ~index):
+        // non-negative floats order like their bit patterns, and on equal
+        // magnitudes the smaller index has the larger complement
         auto publish = [&](int par, float bv, int bi) {
-#pragma unroll
-            for (int o = 32; o > 0; o >>= 1) {
-                const float ov = __shfl_xor(bv, o, 64);
-                const int oi = __shfl_xor(bi, o, 64);
-                if (ov > bv || (ov == bv && oi < bi)) {
-                    bv = ov;
-                    bi = oi;
-                }
+            const unsigned long long key =
+                (bi < m && bv >= 0.0f)
+                    ? (static_cast<unsigned long long>(__float_as_uint(bv)) << 32)
+                          | static_cast<unsigned int>(~bi)
+                    : 0ULL;
+            const unsigned long long kmax = __ockl_wfred_max_u64(key);
+            if (kmax == 0ULL) {
+                bv = -1.0f;
+                bi = m;
+            } else {
+                bv = __uint_as_float(static_cast<unsigned int>(kmax >> 32));
+                bi = ~static_cast<int>(static_cast<unsigned int>(kmax));
             }
             if (lane == 0) {
                 cand_v[par][wave] = bv;

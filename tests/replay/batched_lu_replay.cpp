@@ -27,6 +27,7 @@
 #include <cstdint>
 #include <cstdio>
 #include <cstdlib>
+#include <cstring>
 #include <vector>
 
 namespace {
@@ -214,29 +215,33 @@ void lu_matrix(Checked& M, int n, int* info)
         for (int tid = 0; tid < kThreads; ++tid)
             for (int r = tid; r < m; r += kThreads)
                 for (int c = 0; c < kNB; ++c) pnl.at(r, c) = c < kb ? M.A(k + r, k + c) : 0.0f;
-        // each wave's arg-max candidate (butterfly lane by lane) and a copy
-        // of its row, per column parity
+        // each wave's arg-max candidate (the kernel's one wavefront max of
+        // the key (|v| bits, ~index); no candidate: key 0) and a copy of its
+        // row, per column parity
         auto publish = [&](int par, std::vector<float>& bv, std::vector<int>& bi) {
             for (int w = 0; w < 4; ++w) {
-                for (int o = 32; o > 0; o >>= 1) {
-                    float nv[64];
-                    int ni[64];
-                    for (int l = 0; l < 64; ++l) {
-                        float v = bv[w * 64 + l];
-                        int ii = bi[w * 64 + l];
-                        const float ov = bv[w * 64 + (l ^ o)];
-                        const int oi = bi[w * 64 + (l ^ o)];
-                        if (ov > v || (ov == v && oi < ii)) {
-                            v = ov;
-                            ii = oi;
-                        }
-                        nv[l] = v;
-                        ni[l] = ii;
-                    }
-                    for (int l = 0; l < 64; ++l) {
-                        bv[w * 64 + l] = nv[l];
-                        bi[w * 64 + l] = ni[l];
-                    }
+                unsigned long long kmax = 0;
+                for (int l = 0; l < 64; ++l) {
+                    const float v = bv[w * 64 + l];
+                    const int ii = bi[w * 64 + l];
+                    unsigned int bits;
+                    std::memcpy(&bits, &v, sizeof(bits));
+                    const unsigned long long key =
+                        (ii < m && v >= 0.0f)
+                            ? (static_cast<unsigned long long>(bits) << 32) | static_cast<unsigned int>(~ii)
+                            : 0ULL;
+                    kmax = std::max(kmax, key);
+                }
+                float wv = -1.0f;
+                int wi = m;
+                if (kmax != 0) {
+                    const unsigned int hi = static_cast<unsigned int>(kmax >> 32);
+                    std::memcpy(&wv, &hi, sizeof(wv));
+                    wi = ~static_cast<int>(static_cast<unsigned int>(kmax));
+                }
+                for (int l = 0; l < 64; ++l) {
+                    bv[w * 64 + l] = wv;
+                    bi[w * 64 + l] = wi;
                 }
                 cand_v.at(par, w) = bv[w * 64];
                 cand_i.at(par, w) = bi[w * 64];
