@@ -860,21 +860,24 @@ __global__ __launch_bounds__(1024) void order_kernel(const int* n, int batch, in
 
 // ---------------------------------------------------------------- solves
 
-// t := M^{-1} t for one matrix (t FP32 in global memory): LU: P, unit L,
-// U; Cholesky: L, L^T.  Column-oriented substitutions, two barriers per
-// column.
+// t := M^{-1} t for one matrix (t FP32 in global memory, staged in LDS for
+// the substitutions): LU: P, unit L, U; Cholesky: L, L^T.  Column-oriented
+// substitutions, two barriers per column.
 template <bool LU>
 __device__ void tri_solve(const float* F, long long ldf, int n, const int* ipiv, float* t)
 {
+    __shared__ float ts[kMaxOrder];
     const int tid = threadIdx.x;
+    for (int i = tid; i < n; i += 256) ts[i] = t[i];
+    __syncthreads();
     if (LU) {
         if (tid == 0) {
             for (int j = 0; j < n; ++j) {
                 const int p = ipiv[j] - 1;
                 if (p != j && p >= 0 && p < n) {
-                    const float s = t[j];
-                    t[j] = t[p];
-                    t[p] = s;
+                    const float s = ts[j];
+                    ts[j] = ts[p];
+                    ts[p] = s;
                 }
             }
         }
@@ -882,21 +885,23 @@ __device__ void tri_solve(const float* F, long long ldf, int n, const int* ipiv,
     }
     for (int j = 0; j < n; ++j) {            // forward
         if (!LU) {
-            if (tid == 0) t[j] = t[j] / F[off(j, j, ldf)];
+            if (tid == 0) ts[j] = ts[j] / F[off(j, j, ldf)];
             __syncthreads();
         }
-        const float tj = t[j];
-        for (int i = j + 1 + tid; i < n; i += 256) t[i] -= F[off(i, j, ldf)] * tj;
+        const float tj = ts[j];
+        for (int i = j + 1 + tid; i < n; i += 256) ts[i] -= F[off(i, j, ldf)] * tj;
         __syncthreads();
     }
     for (int j = n - 1; j >= 0; --j) {       // backward
-        if (tid == 0) t[j] = t[j] / F[off(j, j, ldf)];
+        if (tid == 0) ts[j] = ts[j] / F[off(j, j, ldf)];
         __syncthreads();
-        const float tj = t[j];
+        const float tj = ts[j];
         for (int i = tid; i < j; i += 256)
-            t[i] -= (LU ? F[off(i, j, ldf)] : F[off(j, i, ldf)]) * tj;
+            ts[i] -= (LU ? F[off(i, j, ldf)] : F[off(j, i, ldf)]) * tj;
         __syncthreads();
     }
+    for (int i = tid; i < n; i += 256) t[i] = ts[i];
+    __syncthreads();
 }
 
 struct SolveArgs {
